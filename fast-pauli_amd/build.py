@@ -1,0 +1,92 @@
+"""Build recipe for the MI355X library and its host bindings (in-tree).
+
+    python fast-pauli_amd/build.py            # everything
+    python fast-pauli_amd/build.py hip        # libfastpauli_hip.so only
+
+Outputs land in fast-pauli_amd/fast_pauli/ next to the Python package:
+  libfastpauli_hip.so   HIP kernels + C ABI (hipcc, gfx950 only)
+  _fast_pauli*.so       Python surface: pybind11 over the C++ host API, which
+                        reaches the GPU only through the C ABI
+"""
+
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import sysconfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+PKG = os.path.join(HERE, "fast_pauli")
+CSRC = os.path.join(HERE, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+CPP_INCLUDE = os.path.join(HERE, "cpp", "include")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+# Packed FP32 instructions are disabled on purpose: some of their forms have given
+# silently wrong results on this hardware.
+HIP_FLAGS = ["-O3", f"--offload-arch={ARCH}", "-std=c++20", "-fPIC",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wall", "-Wno-unused-result"]
+
+LIB_HIP = os.path.join(PKG, "libfastpauli_hip.so")
+
+
+def _run(cmd: list[str]) -> None:
+    print("+", " ".join(cmd), flush=True)
+    proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    noise = "is not a recognized feature for this target"
+    out = "\n".join(line for line in proc.stdout.splitlines() if noise not in line)
+    if out.strip():
+        print(out, flush=True)
+    if proc.returncode != 0:
+        raise RuntimeError(f"build step failed ({proc.returncode}): {' '.join(cmd)}")
+
+
+def build_hip() -> str:
+    src = os.path.join(CSRC, "fp_hip.hip")
+    tmp = LIB_HIP + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-shared", f"-I{INCLUDE}", f"-I{CSRC}", "-o", tmp, src])
+    os.replace(tmp, LIB_HIP)
+    return LIB_HIP
+
+
+def ext_suffix() -> str:
+    return sysconfig.get_config_var("EXT_SUFFIX") or ".so"
+
+
+def build_python() -> str | None:
+    src = os.path.join(HERE, "cpp", "src", "fast_pauli_py.cpp")
+    if not os.path.exists(src):
+        return None
+    import pybind11
+
+    out = os.path.join(PKG, "_fast_pauli" + ext_suffix())
+    tmp = out + ".tmp"
+    cxx = os.environ.get("CXX", "g++")
+    _run([cxx, "-O3", "-std=c++20", "-fPIC", "-shared", "-Wall", "-Wextra",
+          f"-I{INCLUDE}", f"-I{CPP_INCLUDE}", f"-I{pybind11.get_include()}",
+          f"-I{sysconfig.get_paths()['include']}", "-o", tmp, src,
+          f"-L{PKG}", "-lfastpauli_hip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, out)
+    return out
+
+
+def build_all() -> list[str]:
+    built = [build_hip()]
+    py = build_python()
+    if py:
+        built.append(py)
+    return built
+
+
+if __name__ == "__main__":
+    what = sys.argv[1] if len(sys.argv) > 1 else "all"
+    if what == "hip":
+        build_hip()
+    elif what == "python":
+        build_python()
+    else:
+        build_all()

@@ -1,0 +1,1190 @@
+// fp_hip.hip — runtime and C ABI of libfastpauli_hip.so (gfx950).
+//
+// Implements include/fast_pauli_hip.h: operator sets, per-stream scratch,
+// launch-shape selection and the entry points of the hot path.  Kernels live
+// in fp_kernels.hpp (apply / expectation) and fp_mfma.hpp (contraction).
+#include "fast_pauli_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "fp_kernels.hpp"
+#include "fp_mfma.hpp"
+
+namespace
+{
+
+thread_local std::string g_last_error;
+
+fp_status fail(fp_status code, char const *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define FP_HIP(call)                                                                                                   \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        hipError_t const e_ = (call);                                                                                  \
+        if (e_ != hipSuccess)                                                                                          \
+            return fail(e_ == hipErrorOutOfMemory ? FP_ERR_OUT_OF_MEMORY : FP_ERR_RUNTIME, "%s failed: %s (%s:%d)",    \
+                        #call, hipGetErrorString(e_), __FILE__, __LINE__);                                             \
+    } while (0)
+
+#define FP_LAUNCHED()                                                                                                  \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        hipError_t const e_ = hipGetLastError();                                                                       \
+        if (e_ != hipSuccess)                                                                                          \
+            return fail(FP_ERR_RUNTIME, "kernel launch failed: %s (%s:%d)", hipGetErrorString(e_), __FILE__,          \
+                        __LINE__);                                                                                     \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Per-stream scratch: reduction partials etc.  Grown on demand (after the
+// stream has drained), never shrunk.  Calls on one stream are ordered, so one
+// buffer per stream suffices.
+// ---------------------------------------------------------------------------
+struct Scratch
+{
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+std::mutex g_scratch_mu;
+std::map<std::pair<int, hipStream_t>, Scratch> g_scratch;
+
+fp_status scratch_get(hipStream_t stream, size_t bytes, void **out)
+{
+    int dev = 0;
+    FP_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    Scratch &s = g_scratch[{dev, stream}];
+    if (s.bytes < bytes)
+    {
+        if (s.ptr)
+        {
+            FP_HIP(hipStreamSynchronize(stream));
+            FP_HIP(hipFree(s.ptr));
+            s.ptr = nullptr;
+            s.bytes = 0;
+        }
+        size_t const want = std::max<size_t>(bytes, size_t(1) << 20);
+        FP_HIP(hipMalloc(&s.ptr, want));
+        s.bytes = want;
+    }
+    *out = s.ptr;
+    return FP_OK;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Operator sets
+// ---------------------------------------------------------------------------
+struct fp_hip_ops
+{
+    size_t n_qubits = 0;
+    size_t n_strings = 0;
+    size_t n_operators = 0;
+    int device = 0;
+    std::vector<uint64_t> x, z;
+    std::vector<uint32_t> ny;
+    std::vector<double> coeffs;  // complex128, n_strings
+    std::vector<double> matrix;  // complex128, n_strings x n_operators
+    // device copies
+    uint64_t *d_x = nullptr, *d_z = nullptr;
+    fpk::StringMeta *d_meta = nullptr;
+    double *d_cf128 = nullptr;   // c_s * (-i)^{nY_s}, complex128
+    float *d_cf64 = nullptr;     // same, complex64
+    double *d_scale = nullptr;   // c_s * kappa_s (complex, FP64): PauliOp EV
+    double *d_kappa_c = nullptr; // (kappa_s, 0): PauliString EV (coefficient applied after)
+    double *d_kappa = nullptr;   // kappa_s (real): per-string EV matrix
+    double *d_rowsum128 = nullptr; // SummedPauliOp::apply folded coefficients (complex128)
+    float *d_rowsum64 = nullptr;
+    // contraction operands: W^T as (n_operators x n_strings) real and imaginary parts
+    double *d_wr64 = nullptr, *d_wi64 = nullptr;
+    float *d_wr32 = nullptr, *d_wi32 = nullptr;
+    bool matrix_imag_zero = true;
+};
+
+namespace
+{
+
+void free_device(fp_hip_ops *o)
+{
+    void *ptrs[] = {o->d_x, o->d_z, o->d_meta, o->d_cf128, o->d_cf64, o->d_scale, o->d_kappa_c, o->d_kappa,
+                    o->d_rowsum128, o->d_rowsum64, o->d_wr64, o->d_wi64, o->d_wr32, o->d_wi32};
+    for (void *p : ptrs)
+        if (p)
+            (void)hipFree(p);
+    o->d_x = o->d_z = nullptr;
+    o->d_meta = nullptr;
+    o->d_cf128 = o->d_scale = o->d_kappa_c = o->d_kappa = o->d_rowsum128 = o->d_wr64 = o->d_wi64 = nullptr;
+    o->d_cf64 = o->d_rowsum64 = o->d_wr32 = o->d_wi32 = nullptr;
+}
+
+// (-i)^k as (re, im)
+void minus_i_pow(uint32_t k, double &re, double &im)
+{
+    switch (k & 3u)
+    {
+    case 0:
+        re = 1, im = 0;
+        break;
+    case 1:
+        re = 0, im = -1;
+        break;
+    case 2:
+        re = -1, im = 0;
+        break;
+    default:
+        re = 0, im = 1;
+        break;
+    }
+}
+
+// exact product of c by a unit on an axis (a swap and/or negation)
+void mul_axis_unit(double cr, double ci, double ur, double ui, double &orr, double &oi)
+{
+    if (ui == 0)
+    {
+        orr = ur > 0 ? cr : -cr;
+        oi = ur > 0 ? ci : -ci;
+    }
+    else
+    {
+        // (cr + i ci) * (i ui) = -ci ui + i cr ui
+        orr = ui > 0 ? -ci : ci;
+        oi = ui > 0 ? cr : -cr;
+    }
+}
+
+double kappa_of(uint64_t x, uint32_t ny)
+{
+    if (x == 0)
+        return 1.0;
+    if ((ny & 1u) == 0)
+        return (ny & 2u) ? -2.0 : 2.0;
+    return ((ny & 3u) == 1u) ? 2.0 : -2.0;
+}
+
+template <class T> fp_status upload(T **dptr, std::vector<T> const &h)
+{
+    if (*dptr == nullptr)
+        FP_HIP(hipMalloc(reinterpret_cast<void **>(dptr), std::max<size_t>(h.size(), 1) * sizeof(T)));
+    if (!h.empty())
+        FP_HIP(hipMemcpy(*dptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return FP_OK;
+}
+
+fp_status upload_coeff_tables(fp_hip_ops *o)
+{
+    size_t const N = o->n_strings;
+    std::vector<double> cf128(2 * N), scale(2 * N), kappa_c(2 * N), kappa(N);
+    std::vector<float> cf64(2 * N);
+    for (size_t s = 0; s < N; ++s)
+    {
+        double ur, ui;
+        minus_i_pow(o->ny[s], ur, ui);
+        double const cr = o->coeffs[2 * s], ci = o->coeffs[2 * s + 1];
+        mul_axis_unit(cr, ci, ur, ui, cf128[2 * s], cf128[2 * s + 1]);
+        double fr, fi;
+        mul_axis_unit(double(float(cr)), double(float(ci)), ur, ui, fr, fi);
+        cf64[2 * s] = float(fr);
+        cf64[2 * s + 1] = float(fi);
+        double const k = kappa_of(o->x[s], o->ny[s]);
+        kappa[s] = k;
+        kappa_c[2 * s] = k;
+        kappa_c[2 * s + 1] = 0.0;
+        scale[2 * s] = cr * k;
+        scale[2 * s + 1] = ci * k;
+    }
+    fp_status st;
+    if ((st = upload(&o->d_cf128, cf128)) != FP_OK)
+        return st;
+    if ((st = upload(&o->d_cf64, cf64)) != FP_OK)
+        return st;
+    if ((st = upload(&o->d_scale, scale)) != FP_OK)
+        return st;
+    if ((st = upload(&o->d_kappa_c, kappa_c)) != FP_OK)
+        return st;
+    return upload(&o->d_kappa, kappa);
+}
+
+fp_status check_ops(fp_hip_ops const *ops)
+{
+    if (!ops)
+        return fail(FP_ERR_INVALID_ARGUMENT, "ops is NULL");
+    return FP_OK;
+}
+
+fp_status check_shape(fp_hip_ops const *ops, int dtype, void const *a, void const *b, size_t dim, size_t n_states)
+{
+    fp_status st = check_ops(ops);
+    if (st != FP_OK)
+        return st;
+    if (dtype != FP_C128 && dtype != FP_C64)
+        return fail(FP_ERR_INVALID_ARGUMENT, "unknown dtype %d", dtype);
+    if (ops->n_strings == 0 || ops->n_qubits == 0)
+        return fail(FP_ERR_INVALID_ARGUMENT, "operator has no Pauli strings");
+    size_t const want = size_t(1) << ops->n_qubits;
+    if (dim != want)
+        return fail(FP_ERR_INVALID_ARGUMENT, "states shape (%zu) must match the dimension of the operators (%zu)", dim,
+                    want);
+    if (n_states == 0)
+        return fail(FP_ERR_INVALID_ARGUMENT, "n_states must be positive");
+    if (!a || !b)
+        return fail(FP_ERR_INVALID_ARGUMENT, "NULL data pointer");
+    uintptr_t const align = dtype == FP_C128 ? 16 : 8;
+    if (((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & (align - 1)) != 0)
+        return fail(FP_ERR_INVALID_ARGUMENT, "device buffers must be aligned to the complex element size (%zu bytes)",
+                    size_t(align));
+    return FP_OK;
+}
+
+// A launch stays below 2^32 work-items: at most 2^23 blocks of 256 threads.
+constexpr uint64_t kMaxLaunchBlocks = uint64_t(1) << 23;
+
+// Launch geometry of the vector view of a (dim x n_states) batch.
+struct VecGeom
+{
+    int nc;           // complexes per vector
+    uint64_t vpr;     // vectors per row
+    uint64_t n_vec;   // total vectors
+    bool pow2;
+    uint32_t shift;
+    double inv_vpr;
+};
+
+VecGeom vec_geom(int dtype, size_t dim, size_t n_states, void const *a, void const *b)
+{
+    VecGeom g{};
+    bool const aligned16 = ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & 15u) == 0;
+    g.nc = (dtype == FP_C64 && (n_states % 2 == 0) && aligned16) ? 2 : 1;
+    g.vpr = n_states / g.nc;
+    g.n_vec = uint64_t(dim) * g.vpr;
+    g.pow2 = (g.vpr & (g.vpr - 1)) == 0;
+    g.shift = 0;
+    while ((uint64_t(1) << g.shift) < g.vpr)
+        ++g.shift;
+    g.inv_vpr = 1.0 / double(g.vpr);
+    return g;
+}
+
+template <class T, int NC, int U, bool POW2, bool ACC>
+void launch_string_apply(VecGeom const &g, void const *in, void *out, uint64_t x, uint64_t z, double cr, double ci,
+                         hipStream_t st)
+{
+    typedef typename fpk::VecOf<T, NC>::type V;
+    uint64_t const per_block = uint64_t(fpk::kBlock) * U;
+    for (uint64_t v0 = 0; v0 < g.n_vec; v0 += kMaxLaunchBlocks * per_block)
+    {
+        uint64_t const rest = g.n_vec - v0;
+        uint64_t const blocks = std::min<uint64_t>((rest + per_block - 1) / per_block, kMaxLaunchBlocks);
+        hipLaunchKernelGGL((fpk::k_string_apply<T, NC, U, POW2, ACC>), dim3(uint32_t(blocks)), dim3(fpk::kBlock), 0,
+                           st, static_cast<V const *>(in), static_cast<V *>(out), v0, g.n_vec, g.shift, g.vpr,
+                           g.inv_vpr, x, z, T(cr), T(ci));
+    }
+}
+
+template <class T, int NC>
+void dispatch_string_apply(VecGeom const &g, void const *in, void *out, uint64_t x, uint64_t z, double cr, double ci,
+                           bool acc, hipStream_t st)
+{
+    constexpr int U = 4;
+    if (g.pow2)
+    {
+        if (acc)
+            launch_string_apply<T, NC, U, true, true>(g, in, out, x, z, cr, ci, st);
+        else
+            launch_string_apply<T, NC, U, true, false>(g, in, out, x, z, cr, ci, st);
+    }
+    else
+    {
+        if (acc)
+            launch_string_apply<T, NC, U, false, true>(g, in, out, x, z, cr, ci, st);
+        else
+            launch_string_apply<T, NC, U, false, false>(g, in, out, x, z, cr, ci, st);
+    }
+}
+
+template <class T, int NC, int U, bool POW2, bool ACC>
+void launch_op_apply(VecGeom const &g, void const *in, void *out, uint64_t const *xs, uint64_t const *zs, T const *cf,
+                     uint32_t n_strings, hipStream_t st)
+{
+    typedef typename fpk::VecOf<T, NC>::type V;
+    uint64_t const per_block = uint64_t(fpk::kBlock) * U;
+    for (uint64_t v0 = 0; v0 < g.n_vec; v0 += kMaxLaunchBlocks * per_block)
+    {
+        uint64_t const rest = g.n_vec - v0;
+        uint64_t const blocks = std::min<uint64_t>((rest + per_block - 1) / per_block, kMaxLaunchBlocks);
+        hipLaunchKernelGGL((fpk::k_op_apply<T, NC, U, POW2, ACC>), dim3(uint32_t(blocks)), dim3(fpk::kBlock), 0, st,
+                           static_cast<V const *>(in), static_cast<V *>(out), v0, g.n_vec, g.shift, g.vpr, g.inv_vpr,
+                           xs, zs, cf, n_strings);
+    }
+}
+
+template <class T, int NC>
+void dispatch_op_apply(VecGeom const &g, void const *in, void *out, uint64_t const *xs, uint64_t const *zs,
+                       T const *cf, uint32_t n_strings, bool acc, hipStream_t st)
+{
+    // fewer vectors per thread when the grid would not fill the chip
+    bool const small = g.n_vec < uint64_t(fpk::kBlock) * 2048 * 2;
+    if (small)
+    {
+        if (g.pow2)
+            acc ? launch_op_apply<T, NC, 1, true, true>(g, in, out, xs, zs, cf, n_strings, st)
+                : launch_op_apply<T, NC, 1, true, false>(g, in, out, xs, zs, cf, n_strings, st);
+        else
+            acc ? launch_op_apply<T, NC, 1, false, true>(g, in, out, xs, zs, cf, n_strings, st)
+                : launch_op_apply<T, NC, 1, false, false>(g, in, out, xs, zs, cf, n_strings, st);
+    }
+    else
+    {
+        if (g.pow2)
+            acc ? launch_op_apply<T, NC, 2, true, true>(g, in, out, xs, zs, cf, n_strings, st)
+                : launch_op_apply<T, NC, 2, true, false>(g, in, out, xs, zs, cf, n_strings, st);
+        else
+            acc ? launch_op_apply<T, NC, 2, false, true>(g, in, out, xs, zs, cf, n_strings, st)
+                : launch_op_apply<T, NC, 2, false, false>(g, in, out, xs, zs, cf, n_strings, st);
+    }
+}
+
+// ---- expectation launch shape ----
+struct EvGeom
+{
+    uint32_t ct, rt;  // column threads, row threads
+    uint32_t gx, gy, gz;
+    uint64_t n_pairs, ppb;
+    uint32_t spb;
+};
+
+constexpr int kEvUnroll = 4;
+
+EvGeom ev_geom(VecGeom const &vg, size_t dim, size_t n_strings, bool per_string, size_t n_states)
+{
+    EvGeom e{};
+    e.ct = uint32_t(std::min<uint64_t>(vg.vpr, fpk::kBlock));
+    e.rt = fpk::kBlock / e.ct;
+    e.gy = uint32_t((vg.vpr + e.ct - 1) / e.ct);
+    e.n_pairs = dim / 2;
+    uint64_t const min_ppb = uint64_t(e.rt) * kEvUnroll * 8;
+    uint64_t gx = (e.n_pairs + min_ppb - 1) / min_ppb;
+    uint64_t cap = std::max<uint64_t>(1, 2048 / e.gy);
+    if (per_string)
+    {
+        // keep the (strings x gx x states) FP64 partials modest
+        uint64_t const budget = uint64_t(256) << 20;
+        uint64_t const per_gx = uint64_t(n_strings) * n_states * sizeof(double);
+        cap = std::min<uint64_t>(cap, std::max<uint64_t>(1, budget / std::max<uint64_t>(per_gx, 1)));
+    }
+    gx = std::max<uint64_t>(1, std::min(gx, cap));
+    e.ppb = (e.n_pairs + gx - 1) / gx;
+    e.gx = uint32_t((e.n_pairs + e.ppb - 1) / e.ppb);
+    uint64_t const blocks_xy = uint64_t(e.gx) * e.gy;
+    uint64_t gz = std::min<uint64_t>(n_strings, std::max<uint64_t>(1, (4096 + blocks_xy - 1) / blocks_xy));
+    e.spb = uint32_t((n_strings + gz - 1) / gz);
+    e.gz = uint32_t((n_strings + e.spb - 1) / e.spb);
+    return e;
+}
+
+template <class T, int NC>
+void launch_op_expect(VecGeom const &vg, EvGeom const &e, void const *states, fpk::StringMeta const *meta,
+                      double const *scale, uint32_t n_strings, uint64_t n_states, fpk::Dbl2 *partial, hipStream_t st)
+{
+    typedef typename fpk::VecOf<T, NC>::type V;
+    hipLaunchKernelGGL((fpk::k_op_expect<T, NC, kEvUnroll>), dim3(e.gx, e.gy, e.gz), dim3(fpk::kBlock), 0, st,
+                       static_cast<V const *>(states), vg.vpr, e.ct, e.rt, e.n_pairs, e.ppb, meta, scale, n_strings,
+                       e.spb, n_states, partial);
+}
+
+template <class T, int NC>
+void launch_strings_expect(VecGeom const &vg, EvGeom const &e, void const *states, fpk::StringMeta const *meta,
+                           uint32_t n_strings, uint64_t n_states, double *partial, hipStream_t st)
+{
+    typedef typename fpk::VecOf<T, NC>::type V;
+    hipLaunchKernelGGL((fpk::k_strings_expect<T, NC, kEvUnroll>), dim3(e.gx, e.gy, e.gz), dim3(fpk::kBlock), 0, st,
+                       static_cast<V const *>(states), vg.vpr, e.ct, e.rt, e.n_pairs, e.ppb, meta, n_strings, e.spb,
+                       n_states, partial);
+}
+
+// Expectation values of strings [s_begin, s_begin + count) combined with
+// per-string complex scales, reduced to out[t] (+)= post * sum.
+fp_status run_op_expect(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim, size_t n_states,
+                        size_t s_begin, size_t count, double const *scale_table, double post_re, double post_im,
+                        int accumulate, hipStream_t st)
+{
+    VecGeom const vg = vec_geom(dtype, dim, n_states, states, states);
+    EvGeom const e = ev_geom(vg, dim, count, false, n_states);
+    uint64_t const n_parts = uint64_t(e.gx) * e.gz;
+    void *scratch = nullptr;
+    fp_status s = scratch_get(st, n_parts * n_states * sizeof(fpk::Dbl2), &scratch);
+    if (s != FP_OK)
+        return s;
+    auto *partial = static_cast<fpk::Dbl2 *>(scratch);
+    fpk::StringMeta const *meta = ops->d_meta + s_begin;
+    double const *scale = scale_table + 2 * s_begin;
+    if (dtype == FP_C128)
+        launch_op_expect<double, 1>(vg, e, states, meta, scale, uint32_t(count), n_states, partial, st);
+    else if (vg.nc == 2)
+        launch_op_expect<float, 2>(vg, e, states, meta, scale, uint32_t(count), n_states, partial, st);
+    else
+        launch_op_expect<float, 1>(vg, e, states, meta, scale, uint32_t(count), n_states, partial, st);
+    FP_LAUNCHED();
+    if (dtype == FP_C128)
+        hipLaunchKernelGGL(fpk::k_reduce_partials<double>, dim3(uint32_t(n_states)), dim3(fpk::kBlock), 0, st,
+                           partial, n_parts, uint64_t(n_states), static_cast<double *>(out), accumulate, post_re,
+                           post_im);
+    else
+        hipLaunchKernelGGL(fpk::k_reduce_partials<float>, dim3(uint32_t(n_states)), dim3(fpk::kBlock), 0, st, partial,
+                           n_parts, uint64_t(n_states), static_cast<float *>(out), accumulate, post_re, post_im);
+    FP_LAUNCHED();
+    return FP_OK;
+}
+
+// Scratch bytes of the stage-1 partials of run_strings_expect.
+size_t strings_expect_partial_bytes(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_states, void const *states)
+{
+    VecGeom const vg = vec_geom(dtype, dim, n_states, states, states);
+    EvGeom const e = ev_geom(vg, dim, ops->n_strings, true, n_states);
+    return size_t(e.gx) * ops->n_strings * n_states * sizeof(double);
+}
+
+// Real (n_strings x n_states) matrix of per-string expectation values.
+// `partial` must hold strings_expect_partial_bytes(); NULL = use the stream's scratch.
+fp_status run_strings_expect(fp_hip_ops const *ops, int dtype, void const *states, void *ev, size_t dim,
+                             size_t n_states, hipStream_t st, double *partial)
+{
+    VecGeom const vg = vec_geom(dtype, dim, n_states, states, states);
+    size_t const N = ops->n_strings;
+    EvGeom const e = ev_geom(vg, dim, N, true, n_states);
+    if (!partial)
+    {
+        void *scratch = nullptr;
+        fp_status s = scratch_get(st, size_t(e.gx) * N * n_states * sizeof(double), &scratch);
+        if (s != FP_OK)
+            return s;
+        partial = static_cast<double *>(scratch);
+    }
+    if (dtype == FP_C128)
+        launch_strings_expect<double, 1>(vg, e, states, ops->d_meta, uint32_t(N), n_states, partial, st);
+    else if (vg.nc == 2)
+        launch_strings_expect<float, 2>(vg, e, states, ops->d_meta, uint32_t(N), n_states, partial, st);
+    else
+        launch_strings_expect<float, 1>(vg, e, states, ops->d_meta, uint32_t(N), n_states, partial, st);
+    FP_LAUNCHED();
+    uint64_t const total = uint64_t(N) * n_states;
+    uint32_t const blocks = uint32_t((total + fpk::kBlock - 1) / fpk::kBlock);
+    if (dtype == FP_C128)
+        hipLaunchKernelGGL(fpk::k_reduce_string_partials<double>, dim3(blocks), dim3(fpk::kBlock), 0, st, partial,
+                           e.gx, uint64_t(n_states), uint64_t(N), ops->d_kappa, static_cast<double *>(ev));
+    else
+        hipLaunchKernelGGL(fpk::k_reduce_string_partials<float>, dim3(blocks), dim3(fpk::kBlock), 0, st, partial,
+                           e.gx, uint64_t(n_states), uint64_t(N), ops->d_kappa, static_cast<float *>(ev));
+    FP_LAUNCHED();
+    return FP_OK;
+}
+
+size_t round256(size_t b)
+{
+    return (b + 255) / 256 * 256;
+}
+
+} // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C"
+{
+
+const char *fp_hip_version(void)
+{
+    return "fast_pauli_hip 0.1 (gfx950)";
+}
+
+size_t fp_hip_last_error(char *buf, size_t len)
+{
+    if (buf && len)
+    {
+        size_t const n = std::min(len - 1, g_last_error.size());
+        std::memcpy(buf, g_last_error.data(), n);
+        buf[n] = 0;
+    }
+    return g_last_error.size();
+}
+
+fp_status fp_hip_device_count(int *count)
+{
+    if (!count)
+        return fail(FP_ERR_INVALID_ARGUMENT, "count is NULL");
+    int n = 0;
+    hipError_t const e = hipGetDeviceCount(&n);
+    if (e != hipSuccess)
+    {
+        *count = 0;
+        return fail(FP_ERR_NO_DEVICE, "hipGetDeviceCount failed: %s", hipGetErrorString(e));
+    }
+    *count = n;
+    return FP_OK;
+}
+
+fp_status fp_hip_init(int device)
+{
+    int n = 0;
+    hipError_t const e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0)
+        return fail(FP_ERR_NO_DEVICE, "no HIP device available");
+    if (device < 0 || device >= n)
+        return fail(FP_ERR_INVALID_ARGUMENT, "device %d out of range (%d devices)", device, n);
+    FP_HIP(hipSetDevice(device));
+    FP_HIP(hipFree(nullptr)); // force context creation
+    return FP_OK;
+}
+
+fp_status fp_hip_malloc(void **ptr, size_t bytes)
+{
+    if (!ptr)
+        return fail(FP_ERR_INVALID_ARGUMENT, "ptr is NULL");
+    *ptr = nullptr;
+    FP_HIP(hipMalloc(ptr, std::max<size_t>(bytes, 1)));
+    return FP_OK;
+}
+
+fp_status fp_hip_free(void *ptr)
+{
+    if (ptr)
+        FP_HIP(hipFree(ptr));
+    return FP_OK;
+}
+
+fp_status fp_hip_memcpy(void *dst, void const *src, size_t bytes, int kind, void *stream)
+{
+    if (bytes == 0)
+        return FP_OK;
+    if (!dst || !src)
+        return fail(FP_ERR_INVALID_ARGUMENT, "NULL pointer in memcpy");
+    hipMemcpyKind k = kind == FP_H2D ? hipMemcpyHostToDevice
+                      : kind == FP_D2H ? hipMemcpyDeviceToHost
+                                       : hipMemcpyDeviceToDevice;
+    FP_HIP(hipMemcpyAsync(dst, src, bytes, k, static_cast<hipStream_t>(stream)));
+    if (kind != FP_D2D)
+        FP_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    return FP_OK;
+}
+
+fp_status fp_hip_memset(void *ptr, int value, size_t bytes, void *stream)
+{
+    if (bytes == 0)
+        return FP_OK;
+    FP_HIP(hipMemsetAsync(ptr, value, bytes, static_cast<hipStream_t>(stream)));
+    return FP_OK;
+}
+
+fp_status fp_hip_stream_create(void **stream)
+{
+    if (!stream)
+        return fail(FP_ERR_INVALID_ARGUMENT, "stream is NULL");
+    hipStream_t s = nullptr;
+    FP_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return FP_OK;
+}
+
+fp_status fp_hip_stream_destroy(void *stream)
+{
+    if (!stream)
+        return FP_OK;
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    FP_HIP(hipStreamSynchronize(s));
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        for (auto it = g_scratch.begin(); it != g_scratch.end();)
+        {
+            if (it->first.second == s)
+            {
+                (void)hipFree(it->second.ptr);
+                it = g_scratch.erase(it);
+            }
+            else
+                ++it;
+        }
+    }
+    FP_HIP(hipStreamDestroy(s));
+    return FP_OK;
+}
+
+fp_status fp_hip_stream_synchronize(void *stream)
+{
+    FP_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    return FP_OK;
+}
+
+fp_status fp_hip_device_synchronize(void)
+{
+    FP_HIP(hipDeviceSynchronize());
+    return FP_OK;
+}
+
+fp_status fp_hip_is_device_pointer(void const *ptr, int *is_device)
+{
+    if (!is_device)
+        return fail(FP_ERR_INVALID_ARGUMENT, "is_device is NULL");
+    *is_device = 0;
+    if (!ptr)
+        return FP_OK;
+    hipPointerAttribute_t attr;
+    hipError_t const e = hipPointerGetAttributes(&attr, ptr);
+    if (e != hipSuccess)
+    {
+        (void)hipGetLastError(); // plain host memory: clear the sticky error
+        return FP_OK;
+    }
+    *is_device = attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+    return FP_OK;
+}
+
+fp_status fp_hip_mem_info(size_t *free_bytes, size_t *total_bytes)
+{
+    size_t f = 0, t = 0;
+    FP_HIP(hipMemGetInfo(&f, &t));
+    if (free_bytes)
+        *free_bytes = f;
+    if (total_bytes)
+        *total_bytes = t;
+    return FP_OK;
+}
+
+fp_status fp_hip_event_create(void **event)
+{
+    if (!event)
+        return fail(FP_ERR_INVALID_ARGUMENT, "event is NULL");
+    hipEvent_t e = nullptr;
+    FP_HIP(hipEventCreate(&e));
+    *event = e;
+    return FP_OK;
+}
+
+fp_status fp_hip_event_destroy(void *event)
+{
+    if (event)
+        FP_HIP(hipEventDestroy(static_cast<hipEvent_t>(event)));
+    return FP_OK;
+}
+
+fp_status fp_hip_event_record(void *event, void *stream)
+{
+    if (!event)
+        return fail(FP_ERR_INVALID_ARGUMENT, "event is NULL");
+    FP_HIP(hipEventRecord(static_cast<hipEvent_t>(event), static_cast<hipStream_t>(stream)));
+    return FP_OK;
+}
+
+fp_status fp_hip_event_synchronize(void *event)
+{
+    if (!event)
+        return fail(FP_ERR_INVALID_ARGUMENT, "event is NULL");
+    FP_HIP(hipEventSynchronize(static_cast<hipEvent_t>(event)));
+    return FP_OK;
+}
+
+fp_status fp_hip_event_query(void *event, int *done)
+{
+    if (!event || !done)
+        return fail(FP_ERR_INVALID_ARGUMENT, "NULL event or out-pointer");
+    hipError_t const e = hipEventQuery(static_cast<hipEvent_t>(event));
+    if (e == hipSuccess)
+    {
+        *done = 1;
+        return FP_OK;
+    }
+    if (e == hipErrorNotReady)
+    {
+        (void)hipGetLastError();
+        *done = 0;
+        return FP_OK;
+    }
+    return fail(FP_ERR_RUNTIME, "hipEventQuery failed: %s", hipGetErrorString(e));
+}
+
+fp_status fp_hip_event_elapsed_ms(void *start, void *stop, float *ms)
+{
+    if (!start || !stop || !ms)
+        return fail(FP_ERR_INVALID_ARGUMENT, "NULL event or out-pointer");
+    FP_HIP(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(stop)));
+    return FP_OK;
+}
+
+fp_status fp_hip_fill_states(int dtype, void *states, size_t dim, size_t n_states, uint64_t seed, int normalise,
+                             void *stream)
+{
+    if (!states || dim == 0 || n_states == 0)
+        return fail(FP_ERR_INVALID_ARGUMENT, "empty or NULL states");
+    if (dtype != FP_C128 && dtype != FP_C64)
+        return fail(FP_ERR_INVALID_ARGUMENT, "unknown dtype %d", dtype);
+    hipStream_t const st = static_cast<hipStream_t>(stream);
+    uint64_t const n_elem = uint64_t(dim) * n_states;
+    uint64_t const n_real = 2 * n_elem;
+    uint32_t const blocks = uint32_t((n_real + fpk::kBlock - 1) / fpk::kBlock);
+    if (dtype == FP_C128)
+        hipLaunchKernelGGL(fpk::k_fill_normal<double>, dim3(blocks), dim3(fpk::kBlock), 0, st,
+                           static_cast<double *>(states), n_real, seed);
+    else
+        hipLaunchKernelGGL(fpk::k_fill_normal<float>, dim3(blocks), dim3(fpk::kBlock), 0, st,
+                           static_cast<float *>(states), n_real, seed);
+    FP_LAUNCHED();
+    if (!normalise)
+        return FP_OK;
+    uint64_t const rows_per_block = 1024;
+    uint64_t const n_parts = (dim + rows_per_block - 1) / rows_per_block;
+    void *scratch = nullptr;
+    fp_status s = scratch_get(st, (n_parts + 1) * n_states * sizeof(double), &scratch);
+    if (s != FP_OK)
+        return s;
+    double *partial = static_cast<double *>(scratch);
+    double *inv_norm = partial + n_parts * n_states;
+    dim3 const g1(uint32_t(n_parts), uint32_t((n_states + fpk::kBlock - 1) / fpk::kBlock));
+    if (dtype == FP_C128)
+        hipLaunchKernelGGL(fpk::k_colnorm_partial<double>, g1, dim3(fpk::kBlock), 0, st,
+                           static_cast<double const *>(states), uint64_t(dim), uint64_t(n_states), rows_per_block,
+                           partial);
+    else
+        hipLaunchKernelGGL(fpk::k_colnorm_partial<float>, g1, dim3(fpk::kBlock), 0, st,
+                           static_cast<float const *>(states), uint64_t(dim), uint64_t(n_states), rows_per_block,
+                           partial);
+    FP_LAUNCHED();
+    hipLaunchKernelGGL(fpk::k_colnorm_final, dim3(uint32_t((n_states + fpk::kBlock - 1) / fpk::kBlock)),
+                       dim3(fpk::kBlock), 0, st, partial, n_parts, uint64_t(n_states), inv_norm);
+    FP_LAUNCHED();
+    uint32_t const b2 = uint32_t((n_elem + fpk::kBlock - 1) / fpk::kBlock);
+    if (dtype == FP_C128)
+        hipLaunchKernelGGL(fpk::k_scale_columns<double>, dim3(b2), dim3(fpk::kBlock), 0, st,
+                           static_cast<double *>(states), n_elem, uint64_t(n_states), inv_norm);
+    else
+        hipLaunchKernelGGL(fpk::k_scale_columns<float>, dim3(b2), dim3(fpk::kBlock), 0, st,
+                           static_cast<float *>(states), n_elem, uint64_t(n_states), inv_norm);
+    FP_LAUNCHED();
+    return FP_OK;
+}
+
+fp_status fp_hip_convert(int src_dtype, void const *src, int dst_dtype, void *dst, size_t n_elements, void *stream)
+{
+    if ((src_dtype != FP_C128 && src_dtype != FP_C64) || (dst_dtype != FP_C128 && dst_dtype != FP_C64))
+        return fail(FP_ERR_INVALID_ARGUMENT, "unknown dtype");
+    if (n_elements == 0)
+        return FP_OK;
+    if (!src || !dst)
+        return fail(FP_ERR_INVALID_ARGUMENT, "NULL pointer in convert");
+    hipStream_t const st = static_cast<hipStream_t>(stream);
+    uint64_t const n_real = 2 * uint64_t(n_elements);
+    if (src_dtype == dst_dtype)
+    {
+        FP_HIP(hipMemcpyAsync(dst, src, n_real * (src_dtype == FP_C128 ? 8 : 4), hipMemcpyDeviceToDevice, st));
+        return FP_OK;
+    }
+    if (src == dst)
+        return fail(FP_ERR_INVALID_ARGUMENT, "convert cannot run in place between dtypes");
+    for (uint64_t b = 0; b < n_real; b += kMaxLaunchBlocks * fpk::kBlock)
+    {
+        uint64_t const cnt = std::min<uint64_t>(n_real - b, kMaxLaunchBlocks * fpk::kBlock);
+        uint32_t const blocks = uint32_t((cnt + fpk::kBlock - 1) / fpk::kBlock);
+        if (src_dtype == FP_C128)
+            hipLaunchKernelGGL((fpk::k_convert<double, float>), dim3(blocks), dim3(fpk::kBlock), 0, st,
+                               static_cast<double const *>(src) + b, static_cast<float *>(dst) + b, cnt);
+        else
+            hipLaunchKernelGGL((fpk::k_convert<float, double>), dim3(blocks), dim3(fpk::kBlock), 0, st,
+                               static_cast<float const *>(src) + b, static_cast<double *>(dst) + b, cnt);
+        FP_LAUNCHED();
+    }
+    return FP_OK;
+}
+
+fp_status fp_hip_ops_create(fp_hip_ops **out, size_t n_qubits, size_t n_strings, uint8_t const *codes,
+                            double const *coeffs)
+{
+    if (!out)
+        return fail(FP_ERR_INVALID_ARGUMENT, "ops out-pointer is NULL");
+    *out = nullptr;
+    if (n_qubits == 0 || n_qubits > 62)
+        return fail(FP_ERR_INVALID_ARGUMENT, "n_qubits must be in 1..62 (got %zu)", n_qubits);
+    if (n_strings == 0 || n_strings > 0xffffffffu)
+        return fail(FP_ERR_INVALID_ARGUMENT, "n_strings must be in 1..2^32-1 (got %zu)", n_strings);
+    if (!codes)
+        return fail(FP_ERR_INVALID_ARGUMENT, "codes is NULL");
+    auto *o = new (std::nothrow) fp_hip_ops();
+    if (!o)
+        return fail(FP_ERR_OUT_OF_MEMORY, "host allocation failed");
+    o->n_qubits = n_qubits;
+    o->n_strings = n_strings;
+    (void)hipGetDevice(&o->device);
+    o->x.resize(n_strings);
+    o->z.resize(n_strings);
+    o->ny.resize(n_strings);
+    o->coeffs.assign(2 * n_strings, 0.0);
+    std::vector<fpk::StringMeta> meta(n_strings);
+    for (size_t s = 0; s < n_strings; ++s)
+    {
+        uint64_t x = 0, z = 0;
+        for (size_t i = 0; i < n_qubits; ++i)
+        {
+            uint8_t const c = codes[s * n_qubits + i];
+            if (c > 3)
+            {
+                delete o;
+                return fail(FP_ERR_INVALID_ARGUMENT, "Pauli code must be 0, 1, 2, or 3 (got %u)", unsigned(c));
+            }
+            uint64_t const bit = uint64_t(1) << (n_qubits - 1 - i);
+            if (c == 1 || c == 2)
+                x |= bit;
+            if (c == 2 || c == 3)
+                z |= bit;
+        }
+        o->x[s] = x;
+        o->z[s] = z;
+        o->ny[s] = uint32_t(__builtin_popcountll(x & z));
+        uint32_t h = 0;
+        if (x)
+            h = 63u - uint32_t(__builtin_clzll(x));
+        uint32_t mode = x == 0 ? fpk::kModeSelf : ((o->ny[s] & 1u) ? fpk::kModeIm : fpk::kModeRe);
+        meta[s] = fpk::StringMeta{x, z, h, mode};
+        o->coeffs[2 * s] = coeffs ? coeffs[2 * s] : 1.0;
+        o->coeffs[2 * s + 1] = coeffs ? coeffs[2 * s + 1] : 0.0;
+    }
+    fp_status st;
+    if ((st = upload(&o->d_x, o->x)) != FP_OK || (st = upload(&o->d_z, o->z)) != FP_OK ||
+        (st = upload(&o->d_meta, meta)) != FP_OK || (st = upload_coeff_tables(o)) != FP_OK)
+    {
+        free_device(o);
+        delete o;
+        return st;
+    }
+    *out = o;
+    return FP_OK;
+}
+
+fp_status fp_hip_ops_destroy(fp_hip_ops *ops)
+{
+    if (!ops)
+        return FP_OK;
+    (void)hipDeviceSynchronize();
+    free_device(ops);
+    delete ops;
+    return FP_OK;
+}
+
+fp_status fp_hip_ops_set_coeffs(fp_hip_ops *ops, double const *coeffs)
+{
+    fp_status st = check_ops(ops);
+    if (st != FP_OK)
+        return st;
+    if (!coeffs)
+        return fail(FP_ERR_INVALID_ARGUMENT, "coeffs is NULL");
+    (void)hipDeviceSynchronize();
+    std::memcpy(ops->coeffs.data(), coeffs, 2 * ops->n_strings * sizeof(double));
+    return upload_coeff_tables(ops);
+}
+
+fp_status fp_hip_ops_set_matrix(fp_hip_ops *ops, size_t n_operators, double const *coeffs_nk)
+{
+    fp_status st = check_ops(ops);
+    if (st != FP_OK)
+        return st;
+    if (n_operators == 0 || !coeffs_nk)
+        return fail(FP_ERR_INVALID_ARGUMENT, "empty coefficient matrix");
+    (void)hipDeviceSynchronize();
+    size_t const N = ops->n_strings, K = n_operators;
+    ops->n_operators = K;
+    ops->matrix.assign(coeffs_nk, coeffs_nk + 2 * N * K);
+    // transposed real / imaginary planes (K x N) for the contraction, and the
+    // per-string row sums used by SummedPauliOp::apply
+    std::vector<double> wr(K * N), wi(K * N), rowsum(2 * N, 0.0), rowsum_cf(2 * N);
+    std::vector<float> wr32(K * N), wi32(K * N), rowsum_cf32(2 * N);
+    bool imag_zero = true;
+    for (size_t j = 0; j < N; ++j)
+    {
+        double sr = 0.0, si = 0.0;
+        for (size_t k = 0; k < K; ++k)
+        {
+            double const re = coeffs_nk[2 * (j * K + k)], im = coeffs_nk[2 * (j * K + k) + 1];
+            wr[k * N + j] = re;
+            wi[k * N + j] = im;
+            wr32[k * N + j] = float(re);
+            wi32[k * N + j] = float(im);
+            imag_zero = imag_zero && im == 0.0;
+            sr += re;
+            si += im;
+        }
+        double ur, ui;
+        minus_i_pow(ops->ny[j], ur, ui);
+        mul_axis_unit(sr, si, ur, ui, rowsum_cf[2 * j], rowsum_cf[2 * j + 1]);
+        // complex64: the reference sums the float coefficients in float
+        float fr = 0.f, fi = 0.f;
+        for (size_t k = 0; k < K; ++k)
+        {
+            fr += float(coeffs_nk[2 * (j * K + k)]);
+            fi += float(coeffs_nk[2 * (j * K + k) + 1]);
+        }
+        double gr, gi;
+        mul_axis_unit(double(fr), double(fi), ur, ui, gr, gi);
+        rowsum_cf32[2 * j] = float(gr);
+        rowsum_cf32[2 * j + 1] = float(gi);
+    }
+    ops->matrix_imag_zero = imag_zero;
+    for (void **p : {reinterpret_cast<void **>(&ops->d_wr64), reinterpret_cast<void **>(&ops->d_wi64),
+                     reinterpret_cast<void **>(&ops->d_wr32), reinterpret_cast<void **>(&ops->d_wi32),
+                     reinterpret_cast<void **>(&ops->d_rowsum128), reinterpret_cast<void **>(&ops->d_rowsum64)})
+    {
+        if (*p)
+        {
+            (void)hipFree(*p);
+            *p = nullptr;
+        }
+    }
+    if ((st = upload(&ops->d_wr64, wr)) != FP_OK || (st = upload(&ops->d_wi64, wi)) != FP_OK ||
+        (st = upload(&ops->d_wr32, wr32)) != FP_OK || (st = upload(&ops->d_wi32, wi32)) != FP_OK ||
+        (st = upload(&ops->d_rowsum128, rowsum_cf)) != FP_OK || (st = upload(&ops->d_rowsum64, rowsum_cf32)) != FP_OK)
+        return st;
+    return FP_OK;
+}
+
+fp_status fp_hip_ops_info(fp_hip_ops const *ops, size_t *n_qubits, size_t *n_strings, size_t *n_operators)
+{
+    fp_status st = check_ops(ops);
+    if (st != FP_OK)
+        return st;
+    if (n_qubits)
+        *n_qubits = ops->n_qubits;
+    if (n_strings)
+        *n_strings = ops->n_strings;
+    if (n_operators)
+        *n_operators = ops->n_operators;
+    return FP_OK;
+}
+
+fp_status fp_hip_string_apply_batch(fp_hip_ops const *ops, size_t string_index, int dtype, void const *states,
+                                    void *out, size_t dim, size_t n_states, double c_re, double c_im, int accumulate,
+                                    void *stream)
+{
+    fp_status st = check_shape(ops, dtype, states, out, dim, n_states);
+    if (st != FP_OK)
+        return st;
+    if (string_index >= ops->n_strings)
+        return fail(FP_ERR_INVALID_ARGUMENT, "string index %zu out of range (%zu strings)", string_index,
+                    ops->n_strings);
+    if (states == out)
+        return fail(FP_ERR_INVALID_ARGUMENT, "apply cannot run in place (out aliases states)");
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    VecGeom const g = vec_geom(dtype, dim, n_states, states, out);
+    // fold (-i)^{nY} into the coefficient; exact
+    double ur, ui, cr, ci;
+    minus_i_pow(ops->ny[string_index], ur, ui);
+    if (dtype == FP_C64)
+    {
+        c_re = double(float(c_re));
+        c_im = double(float(c_im));
+    }
+    mul_axis_unit(c_re, c_im, ur, ui, cr, ci);
+    uint64_t const x = ops->x[string_index], z = ops->z[string_index];
+    if (dtype == FP_C128)
+        dispatch_string_apply<double, 1>(g, states, out, x, z, cr, ci, accumulate != 0, s);
+    else if (g.nc == 2)
+        dispatch_string_apply<float, 2>(g, states, out, x, z, cr, ci, accumulate != 0, s);
+    else
+        dispatch_string_apply<float, 1>(g, states, out, x, z, cr, ci, accumulate != 0, s);
+    FP_LAUNCHED();
+    return FP_OK;
+}
+
+fp_status fp_hip_string_expectation(fp_hip_ops const *ops, size_t string_index, int dtype, void const *states,
+                                    void *out, size_t dim, size_t n_states, double c_re, double c_im, int accumulate,
+                                    void *stream)
+{
+    fp_status st = check_shape(ops, dtype, states, out, dim, n_states);
+    if (st != FP_OK)
+        return st;
+    if (string_index >= ops->n_strings)
+        return fail(FP_ERR_INVALID_ARGUMENT, "string index %zu out of range (%zu strings)", string_index,
+                    ops->n_strings);
+    return run_op_expect(ops, dtype, states, out, dim, n_states, string_index, 1, ops->d_kappa_c, c_re, c_im,
+                         accumulate, static_cast<hipStream_t>(stream));
+}
+
+fp_status fp_hip_op_apply(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim,
+                          size_t n_states, int accumulate, void *stream)
+{
+    fp_status st = check_shape(ops, dtype, states, out, dim, n_states);
+    if (st != FP_OK)
+        return st;
+    if (states == out)
+        return fail(FP_ERR_INVALID_ARGUMENT, "apply cannot run in place (out aliases states)");
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    VecGeom const g = vec_geom(dtype, dim, n_states, states, out);
+    uint32_t const N = uint32_t(ops->n_strings);
+    if (dtype == FP_C128)
+        dispatch_op_apply<double, 1>(g, states, out, ops->d_x, ops->d_z, ops->d_cf128, N, accumulate != 0, s);
+    else if (g.nc == 2)
+        dispatch_op_apply<float, 2>(g, states, out, ops->d_x, ops->d_z, ops->d_cf64, N, accumulate != 0, s);
+    else
+        dispatch_op_apply<float, 1>(g, states, out, ops->d_x, ops->d_z, ops->d_cf64, N, accumulate != 0, s);
+    FP_LAUNCHED();
+    return FP_OK;
+}
+
+fp_status fp_hip_op_expectation(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim,
+                                size_t n_states, int accumulate, void *stream)
+{
+    fp_status st = check_shape(ops, dtype, states, out, dim, n_states);
+    if (st != FP_OK)
+        return st;
+    return run_op_expect(ops, dtype, states, out, dim, n_states, 0, ops->n_strings, ops->d_scale, 1.0, 0.0,
+                         accumulate, static_cast<hipStream_t>(stream));
+}
+
+fp_status fp_hip_strings_expectation_matrix(fp_hip_ops const *ops, int dtype, void const *states, void *ev,
+                                            size_t dim, size_t n_states, void *stream)
+{
+    fp_status st = check_shape(ops, dtype, states, ev, dim, n_states);
+    if (st != FP_OK)
+        return st;
+    return run_strings_expect(ops, dtype, states, ev, dim, n_states, static_cast<hipStream_t>(stream), nullptr);
+}
+
+fp_status fp_hip_summed_expectation(fp_hip_ops const *ops, int dtype, int contraction, void const *states, void *out,
+                                    size_t dim, size_t n_states, int accumulate, void *stream)
+{
+    fp_status st = check_shape(ops, dtype, states, out, dim, n_states);
+    if (st != FP_OK)
+        return st;
+    if (ops->n_operators == 0)
+        return fail(FP_ERR_INVALID_ARGUMENT, "SummedPauliOp has no coefficient matrix");
+    if (contraction == FP_CONTRACT_AUTO)
+        contraction = dtype == FP_C128 ? FP_CONTRACT_FP64 : FP_CONTRACT_FP32;
+    if (contraction < FP_CONTRACT_FP64 || contraction > FP_CONTRACT_BF16X3)
+        return fail(FP_ERR_INVALID_ARGUMENT, "unknown contraction mode %d", contraction);
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    size_t const N = ops->n_strings, K = ops->n_operators, S = n_states;
+    bool const fp64 = contraction == FP_CONTRACT_FP64;
+    size_t const esz = fp64 ? sizeof(double) : sizeof(float);
+    size_t const nat_esz = dtype == FP_C128 ? sizeof(double) : sizeof(float);
+    bool const convert = esz != nat_esz;
+    // scratch: [ev at the states' precision | ev converted (if needed) | C (planes x K x S) | stage-1 partials]
+    size_t const ev_nat_bytes = round256(N * S * nat_esz);
+    size_t const ev_conv_bytes = convert ? round256(N * S * esz) : 0;
+    size_t const c_bytes = round256(2 * K * S * esz);
+    size_t const part_bytes = strings_expect_partial_bytes(ops, dtype, dim, S, states);
+    void *scratch = nullptr;
+    if ((st = scratch_get(s, ev_nat_bytes + ev_conv_bytes + c_bytes + part_bytes, &scratch)) != FP_OK)
+        return st;
+    char *base = static_cast<char *>(scratch);
+    void *ev_nat = base;
+    void *ev = convert ? static_cast<void *>(base + ev_nat_bytes) : ev_nat;
+    void *cmat = base + ev_nat_bytes + ev_conv_bytes;
+    auto *partial = reinterpret_cast<double *>(base + ev_nat_bytes + ev_conv_bytes + c_bytes);
+    if ((st = run_strings_expect(ops, dtype, states, ev_nat, dim, S, s, partial)) != FP_OK)
+        return st;
+    if (convert)
+    {
+        uint32_t const cb = uint32_t((N * S + 255) / 256);
+        if (dtype == FP_C128)
+            hipLaunchKernelGGL((fpk::k_convert<double, float>), dim3(cb), dim3(256), 0, s,
+                               static_cast<double const *>(ev_nat), static_cast<float *>(ev), uint64_t(N * S));
+        else
+            hipLaunchKernelGGL((fpk::k_convert<float, double>), dim3(cb), dim3(256), 0, s,
+                               static_cast<float const *>(ev_nat), static_cast<double *>(ev), uint64_t(N * S));
+        FP_LAUNCHED();
+    }
+    // stage 2: C_re = Wr^T ev, C_im = Wi^T ev  (ev is real: Pauli strings are Hermitian)
+    size_t const planes = ops->matrix_imag_zero ? 1 : 2;
+    for (size_t p = 0; p < planes; ++p)
+    {
+        void const *w = fp64 ? static_cast<void const *>(p == 0 ? ops->d_wr64 : ops->d_wi64)
+                             : static_cast<void const *>(p == 0 ? ops->d_wr32 : ops->d_wi32);
+        void *c = static_cast<char *>(cmat) + p * K * S * esz;
+        if ((st = fpm::gemm(contraction, K, S, N, w, ev, c, s)) != FP_OK)
+            return fail(st, "contraction failed: %s", fpm::last_error());
+    }
+    uint64_t const total = uint64_t(K) * S;
+    uint32_t const blocks = uint32_t((total + 255) / 256);
+    int const imag_zero = planes == 1;
+    if (fp64)
+    {
+        if (dtype == FP_C128)
+            hipLaunchKernelGGL((fpk::k_pack_complex<double, double>), dim3(blocks), dim3(256), 0, s,
+                               static_cast<double const *>(cmat), static_cast<double *>(out), total, imag_zero,
+                               accumulate);
+        else
+            hipLaunchKernelGGL((fpk::k_pack_complex<double, float>), dim3(blocks), dim3(256), 0, s,
+                               static_cast<double const *>(cmat), static_cast<float *>(out), total, imag_zero,
+                               accumulate);
+    }
+    else
+    {
+        if (dtype == FP_C128)
+            hipLaunchKernelGGL((fpk::k_pack_complex<float, double>), dim3(blocks), dim3(256), 0, s,
+                               static_cast<float const *>(cmat), static_cast<double *>(out), total, imag_zero,
+                               accumulate);
+        else
+            hipLaunchKernelGGL((fpk::k_pack_complex<float, float>), dim3(blocks), dim3(256), 0, s,
+                               static_cast<float const *>(cmat), static_cast<float *>(out), total, imag_zero,
+                               accumulate);
+    }
+    FP_LAUNCHED();
+    return FP_OK;
+}
+
+fp_status fp_hip_summed_apply(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim,
+                              size_t n_states, int accumulate, void *stream)
+{
+    fp_status st = check_shape(ops, dtype, states, out, dim, n_states);
+    if (st != FP_OK)
+        return st;
+    if (ops->n_operators == 0)
+        return fail(FP_ERR_INVALID_ARGUMENT, "SummedPauliOp has no coefficient matrix");
+    if (states == out)
+        return fail(FP_ERR_INVALID_ARGUMENT, "apply cannot run in place (out aliases states)");
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    VecGeom const g = vec_geom(dtype, dim, n_states, states, out);
+    uint32_t const N = uint32_t(ops->n_strings);
+    if (dtype == FP_C128)
+        dispatch_op_apply<double, 1>(g, states, out, ops->d_x, ops->d_z, ops->d_rowsum128, N, accumulate != 0, s);
+    else if (g.nc == 2)
+        dispatch_op_apply<float, 2>(g, states, out, ops->d_x, ops->d_z, ops->d_rowsum64, N, accumulate != 0, s);
+    else
+        dispatch_op_apply<float, 1>(g, states, out, ops->d_x, ops->d_z, ops->d_rowsum64, N, accumulate != 0, s);
+    FP_LAUNCHED();
+    return FP_OK;
+}
+
+fp_status fp_hip_summed_apply_weighted(fp_hip_ops const *, int, void const *, void const *, void *, size_t, size_t,
+                                       int, void *)
+{
+    return fail(FP_ERR_UNSUPPORTED, "SummedPauliOp::apply_weighted is not implemented on the GPU yet");
+}
+
+fp_status fp_hip_gemm(int contraction, size_t m, size_t n, size_t k, void const *a, void const *b, void *c,
+                      void *stream)
+{
+    if (!a || !b || !c || m == 0 || n == 0 || k == 0)
+        return fail(FP_ERR_INVALID_ARGUMENT, "empty or NULL GEMM operand");
+    fp_status const st = fpm::gemm(contraction, m, n, k, a, b, c, static_cast<hipStream_t>(stream));
+    if (st != FP_OK)
+        return fail(st, "%s", fpm::last_error());
+    return FP_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,189 @@
+/*
+ * fast_pauli_hip.h — C ABI of the MI355X (gfx950) operator-application library
+ * (libfastpauli_hip.so).
+ *
+ * This is the drop-in boundary.  Everything above it (the C++ host API in
+ * fast-pauli_amd/cpp/include/fast_pauli/ and the Python package built on it)
+ * reaches the HIP kernels only through these functions.  Signatures use plain
+ * pointers and sizes; no HIP, torch or C++ types cross the boundary and no C++
+ * exception escapes it.  Every function returns an fp_status; the message of
+ * the last failure on the calling thread is available from
+ * fp_hip_last_error().
+ *
+ * Conventions
+ *  - A batch of states is a row-major (dim x n_states) complex array, element
+ *    (j, t) at j * n_states + t, exactly the C-contiguous layout the reference
+ *    enforces (fast_pauli/cpp/include/__nb_helpers.hpp:55-74).  dim = 2^n.
+ *  - Complex values are interleaved (re, im): FP_C128 = two doubles,
+ *    FP_C64 = two floats.
+ *  - Pauli strings are given as letter codes (0 = I, 1 = X, 2 = Y, 3 = Z,
+ *    fast_pauli/cpp/include/__pauli.hpp:41), n_qubits codes per string,
+ *    character 0 first; character i acts on bit (n-1-i) of the row index
+ *    (fast_pauli/cpp/include/__pauli_string.hpp:52-54).
+ *  - Compute entry points take DEVICE pointers and are asynchronous on the
+ *    given HIP stream (NULL = the null stream).  `accumulate` = 1 adds into
+ *    the output (out += result, the reference C++ API's semantics); 0
+ *    overwrites it (what the reference's Python layer gets from its fresh,
+ *    zeroed outputs).
+ *  - Library-owned scratch (reduction partials) is kept per stream.  Calls
+ *    on one stream are ordered; calls on different streams may overlap.
+ */
+#ifndef FAST_PAULI_HIP_H
+#define FAST_PAULI_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int fp_status;
+enum
+{
+    FP_OK = 0,
+    FP_ERR_INVALID_ARGUMENT = 1, /* maps to std::invalid_argument / ValueError */
+    FP_ERR_RUNTIME = 2,          /* HIP runtime failure */
+    FP_ERR_OUT_OF_MEMORY = 3,
+    FP_ERR_NO_DEVICE = 4,
+    FP_ERR_UNSUPPORTED = 5
+};
+
+/* element types */
+enum
+{
+    FP_C128 = 0, /* complex128, FP64 arithmetic (the reference's precision) */
+    FP_C64 = 1   /* complex64, FP32 arithmetic with FP32 accumulation and an FP64 final cross-row reduction */
+};
+
+/* precision of SummedPauliOp's (operators x strings) x (strings x states) contraction */
+enum
+{
+    FP_CONTRACT_FP64 = 0,      /* FP64 MFMA (plain / oracle mode) */
+    FP_CONTRACT_FP32 = 1,      /* FP32 MFMA */
+    FP_CONTRACT_BF16X3 = 2,    /* BF16 split-precision MFMA: hi*hi + hi*lo + lo*hi, FP32 accumulation */
+    FP_CONTRACT_AUTO = 3       /* FP64 for FP_C128 states, FP32 for FP_C64 */
+};
+
+/* memcpy kinds */
+enum
+{
+    FP_H2D = 0,
+    FP_D2H = 1,
+    FP_D2D = 2
+};
+
+typedef struct fp_hip_ops fp_hip_ops; /* device-resident set of Pauli strings and coefficients */
+
+/* ---- library and device --------------------------------------------------- */
+const char *fp_hip_version(void);
+fp_status fp_hip_init(int device);                 /* selects and initialises `device` */
+fp_status fp_hip_device_count(int *count);
+size_t fp_hip_last_error(char *buf, size_t len);   /* copies the last message; returns its full length */
+
+/* ---- memory and streams (so callers need no HIP headers) ------------------ */
+fp_status fp_hip_malloc(void **ptr, size_t bytes);
+fp_status fp_hip_free(void *ptr);
+fp_status fp_hip_memcpy(void *dst, void const *src, size_t bytes, int kind, void *stream);
+fp_status fp_hip_memset(void *ptr, int value, size_t bytes, void *stream);
+fp_status fp_hip_stream_create(void **stream);
+fp_status fp_hip_stream_destroy(void *stream);
+fp_status fp_hip_stream_synchronize(void *stream);
+fp_status fp_hip_device_synchronize(void);
+fp_status fp_hip_is_device_pointer(void const *ptr, int *is_device);
+fp_status fp_hip_mem_info(size_t *free_bytes, size_t *total_bytes);
+/* HIP events, for timing work on the stream it is launched on */
+fp_status fp_hip_event_create(void **event);
+fp_status fp_hip_event_destroy(void *event);
+fp_status fp_hip_event_record(void *event, void *stream);
+fp_status fp_hip_event_synchronize(void *event);
+fp_status fp_hip_event_query(void *event, int *done); /* done = 1 once the work before the record has finished */
+fp_status fp_hip_event_elapsed_ms(void *start, void *stop, float *ms);
+
+/* ---- seeded synthetic inputs ------------------------------------------------
+ * Fills a (dim x n_states) batch with re/im ~ N(0,1) from counter-based
+ * splitmix64 + Box-Muller (element e, component c uses stream index 2e+c) and,
+ * if normalise != 0, scales every column to unit 2-norm (FP64 norms). */
+fp_status fp_hip_fill_states(int dtype, void *states, size_t dim, size_t n_states, uint64_t seed, int normalise,
+                             void *stream);
+
+/* Element-wise conversion between complex128 and complex64 batches on the device
+ * (round to nearest when narrowing).  n_elements complex values. */
+fp_status fp_hip_convert(int src_dtype, void const *src, int dst_dtype, void *dst, size_t n_elements, void *stream);
+
+/* ---- operator sets ------------------------------------------------------------
+ * A set of n_strings Pauli strings on n_qubits (1..62) qubits with one complex128
+ * coefficient per string (NULL = all ones).  Replaces the per-call
+ * get_sparse_repr materialisation (fast_pauli/cpp/include/__pauli_string.hpp:49-118):
+ * the kernels evaluate column index j^x and phase (-i)^{nY} (-1)^{popcount(j&z)}
+ * from per-string bit masks instead. */
+fp_status fp_hip_ops_create(fp_hip_ops **ops, size_t n_qubits, size_t n_strings, uint8_t const *codes,
+                            double const *coeffs);
+fp_status fp_hip_ops_destroy(fp_hip_ops *ops);
+/* Replace the per-string coefficients (PauliOp::scale, extend, ...). */
+fp_status fp_hip_ops_set_coeffs(fp_hip_ops *ops, double const *coeffs);
+/* Attach SummedPauliOp's coefficient matrix h(j, k): complex128, (n_strings x n_operators) row-major,
+ * the layout of fast_pauli/cpp/include/__summed_pauli_op.hpp:45. */
+fp_status fp_hip_ops_set_matrix(fp_hip_ops *ops, size_t n_operators, double const *coeffs_nk);
+fp_status fp_hip_ops_info(fp_hip_ops const *ops, size_t *n_qubits, size_t *n_strings, size_t *n_operators);
+
+/* ---- hot path ---------------------------------------------------------------- */
+
+/* PauliString::apply_batch (fast_pauli/cpp/include/__pauli_string.hpp:396-436) and, with
+ * n_states = 1, PauliString::apply (:312-341): out(j,t) (+)= c * P_s psi, P_s = string `string_index`.
+ * Bit-exact with the reference's FP arithmetic (no fused multiply-add). */
+fp_status fp_hip_string_apply_batch(fp_hip_ops const *ops, size_t string_index, int dtype, void const *states,
+                                    void *out, size_t dim, size_t n_states, double c_re, double c_im,
+                                    int accumulate, void *stream);
+
+/* PauliString::expectation_value (fast_pauli/cpp/include/__pauli_string.hpp:488-538):
+ * out[t] (+)= c * <psi_t| P_s |psi_t>, out has n_states complex elements of `dtype`. */
+fp_status fp_hip_string_expectation(fp_hip_ops const *ops, size_t string_index, int dtype, void const *states,
+                                    void *out, size_t dim, size_t n_states, double c_re, double c_im,
+                                    int accumulate, void *stream);
+
+/* PauliOp::apply (fast_pauli/cpp/include/__pauli_op.hpp:376-468): out (+)= sum_s h_s P_s psi.
+ * Strings are accumulated in order s = 0..N-1 per element, which is the reference's
+ * sequential order, so results are bit-exact with it. */
+fp_status fp_hip_op_apply(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim,
+                          size_t n_states, int accumulate, void *stream);
+
+/* PauliOp::expectation_value (fast_pauli/cpp/include/__pauli_op.hpp:496-549):
+ * out[t] (+)= sum_s h_s <psi_t| P_s |psi_t>. */
+fp_status fp_hip_op_expectation(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim,
+                                size_t n_states, int accumulate, void *stream);
+
+/* Per-string expectation values without coefficients, as a REAL (n_strings x n_states) matrix
+ * ev(j,t) = <psi_t|P_j|psi_t> (FP64 for FP_C128, FP32 for FP_C64).  Pauli strings are Hermitian,
+ * so the imaginary part is exactly zero.  Stage 1 of SummedPauliOp::expectation_value
+ * (fast_pauli/cpp/include/__summed_pauli_op.hpp:572-577). */
+fp_status fp_hip_strings_expectation_matrix(fp_hip_ops const *ops, int dtype, void const *states, void *ev,
+                                            size_t dim, size_t n_states, void *stream);
+
+/* SummedPauliOp::expectation_value (fast_pauli/cpp/include/__summed_pauli_op.hpp:530-614):
+ * out(k,t) (+)= sum_j h(j,k) <psi_t|P_j|psi_t>; out is (n_operators x n_states) of `dtype`.
+ * `contraction` selects the MFMA precision of stage 2 (FP_CONTRACT_*). */
+fp_status fp_hip_summed_expectation(fp_hip_ops const *ops, int dtype, int contraction, void const *states,
+                                    void *out, size_t dim, size_t n_states, int accumulate, void *stream);
+
+/* SummedPauliOp::apply (fast_pauli/cpp/include/__summed_pauli_op.hpp:287-349):
+ * out (+)= sum_j (sum_k h(j,k)) P_j psi. */
+fp_status fp_hip_summed_apply(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim,
+                              size_t n_states, int accumulate, void *stream);
+
+/* SummedPauliOp::apply_weighted (fast_pauli/cpp/include/__summed_pauli_op.hpp:378-503):
+ * out(:,t) (+)= sum_j (sum_k h(j,k) x(k,t)) P_j psi(:,t); `data` x is a real (n_operators x n_states)
+ * row-major device array (double for FP_C128, float for FP_C64). */
+fp_status fp_hip_summed_apply_weighted(fp_hip_ops const *ops, int dtype, void const *states, void const *data,
+                                       void *out, size_t dim, size_t n_states, int accumulate, void *stream);
+
+/* ---- real GEMM on MFMA (stage 2 of SummedPauliOp, exposed for testing) ---------
+ * C(MxN) = A(MxK) * B(KxN), all row-major, FP64 for FP_CONTRACT_FP64 else FP32 operands/results. */
+fp_status fp_hip_gemm(int contraction, size_t m, size_t n, size_t k, void const *a, void const *b, void *c,
+                      void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FAST_PAULI_HIP_H */

@@ -1,0 +1,274 @@
+"""GPU parity of the hot path, through the C ABI, against the CPU oracle.
+
+Bars:
+  * apply / apply_batch / PauliOp::apply: bit-exact with the oracle's
+    sequential loops (same FP operations, no fused multiply-add), in both
+    complex128 and complex64.  Bit-exact here means IEEE equality
+    (np.array_equal); only the sign of an exact zero could differ.
+  * expectation values, complex128: <= 1e-12 normwise relative to the oracle.
+  * expectation values, complex64 (FP32 accumulation): <= 1e-5 normwise
+    relative to the FP64 oracle evaluated on the same complex64 inputs.
+"""
+
+import itertools as it
+
+import numpy as np
+import pytest
+
+from fast_pauli import _abi
+from fast_pauli import synthetic as syn
+from oracle import oracle as O
+from tests import hiputil as H
+
+pytestmark = pytest.mark.gpu
+
+C128, C64 = _abi.FP_C128, _abi.FP_C64
+SAMPLE = (["I", "X", "Y", "Z"] + ["".join(p) for p in it.product("IXYZ", repeat=2)]
+          + ["".join(p) for p in it.product("IXYZ", repeat=3)]
+          + ["XYZXYZ", "ZZZIII", "XYIZXYZ", "XXIYYIZZ", "ZIXIZYXX"])
+
+
+def ops_of(strings, coeffs=None):
+    return _abi.Ops(O.codes_of(strings), coeffs)
+
+
+def odt(dtype):
+    return O.C128 if dtype == C128 else O.C64
+
+
+# ---------------------------------------------------------------------------
+# PauliString::apply_batch / apply
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("dtype", [C128, C64])
+def test_string_apply_batch_sample_bit_exact(hip, dtype):
+    cdt = H.CDT[dtype]
+    for n_states in (1, 3, 16, 42):
+        for s in SAMPLE:
+            H.say(f"string_apply {s} S={n_states} dtype={dtype}")
+            ops = ops_of([s])
+            psi = syn.random_states(1 << len(s), n_states, seed=len(s) + n_states).astype(cdt)
+            for c in (1.0, 0.3 - 1.7j):
+                got = H.string_apply(ops, 0, psi, c, dtype)
+                exp = O.string_apply_batch(s, c, psi, np.zeros_like(psi), dtype=odt(dtype))
+                assert np.array_equal(got, exp), (s, n_states, c)
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
+@pytest.mark.parametrize("n,S", [(10, 1), (10, 16), (12, 7), (14, 128), (9, 1000)])
+def test_string_apply_batch_random_bit_exact(hip, dtype, n, S):
+    cdt = H.CDT[dtype]
+    codes = syn.random_codes(6, n, seed=n * 31 + S)
+    strings = syn.codes_to_strings(codes)
+    ops = _abi.Ops(codes)
+    psi = syn.random_states(1 << n, S, seed=S).astype(cdt)
+    base = syn.random_states(1 << n, S, seed=S + 1).astype(cdt)
+    for i, s in enumerate(strings):
+        H.say(f"string_apply random n={n} S={S} idx={i} dtype={dtype}")
+        c = complex(syn.random_coeffs(1, seed=i)[0])
+        got = H.string_apply(ops, i, psi, c, dtype)
+        exp = O.string_apply_batch(s, c, psi, np.zeros_like(psi), dtype=odt(dtype))
+        assert np.array_equal(got, exp)
+        # accumulate (+=) semantics of the C++ API
+        got = H.string_apply(ops, i, psi, c, dtype, out0=base)
+        exp = O.string_apply_batch(s, c, psi, base.copy(), dtype=odt(dtype))
+        assert np.array_equal(got, exp)
+
+
+def test_string_apply_1d_known_answers(hip):
+    H.say("string_apply known answers")
+    # test_pauli_string.py:184-198 (1-D inputs are the S = 1 case)
+    got = H.string_apply(ops_of(["IXYZ"]), 0, np.zeros((16, 1)), 1.0)
+    assert np.array_equal(got, np.zeros((16, 1)))
+    a = np.arange(8).astype(np.complex128).reshape(8, 1)
+    assert np.array_equal(H.string_apply(ops_of(["III"]), 0, a), a)
+    got = H.string_apply(ops_of(["ZYX"]), 0, np.ones((8, 1)))
+    np.testing.assert_array_equal(got[:, 0], O.dense_string("ZYX").sum(1))
+    got = H.string_apply(ops_of(["ZYX"]), 0, np.eye(8))
+    np.testing.assert_array_equal(got, O.dense_string("ZYX"))
+
+
+# ---------------------------------------------------------------------------
+# PauliOp::apply
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("dtype", [C128, C64])
+@pytest.mark.parametrize("n,N,S", [(1, 3, 1), (5, 7, 5), (10, 64, 128), (12, 33, 3), (8, 512, 2)])
+def test_op_apply_bit_exact(hip, dtype, n, N, S):
+    H.say(f"op_apply n={n} N={N} S={S} dtype={dtype}")
+    cdt = H.CDT[dtype]
+    codes = syn.random_codes(N, n, seed=N + n)
+    strings = syn.codes_to_strings(codes)
+    coeffs = syn.random_coeffs(N, seed=N)
+    ops = _abi.Ops(codes, coeffs)
+    psi = syn.random_states(1 << n, S, seed=n).astype(cdt)
+    got = H.op_apply(ops, psi, dtype)
+    exp = O.op_apply(strings, coeffs.astype(cdt), psi, np.zeros_like(psi), dtype=odt(dtype))
+    assert np.array_equal(got, exp)
+    base = syn.random_states(1 << n, S, seed=n + 7).astype(cdt)
+    got = H.op_apply(ops, psi, dtype, out0=base)
+    exp = O.op_apply(strings, coeffs.astype(cdt), psi, base.copy(), dtype=odt(dtype))
+    assert np.array_equal(got, exp)
+
+
+def test_op_apply_known_answers(hip):
+    H.say("op_apply known answers")
+    a = np.arange(64).astype(np.complex128).reshape(8, 8)
+    got = H.op_apply(ops_of(["III", "III"], [0.5, 0.5]), a)
+    np.testing.assert_array_equal(got, a)  # test_pauli_op.py:234-236
+    got = H.op_apply(ops_of(["XYZ"], [1j]), np.eye(8))
+    np.testing.assert_allclose(got, 1j * O.dense_string("XYZ"), atol=1e-15)
+    got = H.op_apply(ops_of(["IXYZ", "ZYXI"], [1, 2]), np.zeros((16, 16)))
+    assert np.array_equal(got, np.zeros((16, 16)))
+
+
+# ---------------------------------------------------------------------------
+# expectation values
+# ---------------------------------------------------------------------------
+def test_string_expectation_sample(hip):
+    for s in SAMPLE:
+        H.say(f"string_expectation {s}")
+        psi = syn.random_states(1 << len(s), 21, seed=len(s))
+        ops = ops_of([s])
+        for c in (1.0, 2.5 - 0.5j):
+            got = H.string_expectation(ops, 0, psi, c)
+            exp = O.string_expectation(s, c, psi, np.zeros(21, np.complex128))
+            np.testing.assert_allclose(got, exp, rtol=0, atol=1e-14)
+    # known answer: <arange|III|arange> = 140 (test_pauli_string.py:285-289)
+    got = H.string_expectation(ops_of(["III"]), 0, np.arange(8).reshape(8, 1))
+    np.testing.assert_allclose(got, [140], rtol=1e-15)
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
+@pytest.mark.parametrize("n,N,S", [(1, 4, 1), (6, 30, 9), (10, 64, 16), (14, 256, 16), (12, 100, 1000)])
+def test_op_expectation(hip, dtype, n, N, S):
+    H.say(f"op_expectation n={n} N={N} S={S} dtype={dtype}")
+    cdt = H.CDT[dtype]
+    codes = syn.random_codes(N, n, seed=3 * N + n)
+    strings = syn.codes_to_strings(codes)
+    coeffs = syn.random_coeffs(N, seed=N + 1)
+    ops = _abi.Ops(codes, coeffs)
+    psi = syn.random_states(1 << n, S, seed=n + S).astype(cdt)
+    got = H.op_expectation(ops, psi, dtype)
+    # FP64 oracle on the same inputs
+    exp = O.op_expectation(strings, coeffs.astype(cdt).astype(np.complex128),
+                           psi.astype(np.complex128), np.zeros(S, np.complex128))
+    tol = 1e-12 if dtype == C128 else 1e-5
+    err = H.rel_err(got, exp)
+    assert err <= tol, err
+    assert H.ev_scaled_err(got, exp, coeffs, psi) <= tol
+    base = syn.random_coeffs(S, seed=99).astype(cdt)
+    got2 = H.op_expectation(ops, psi, dtype, out0=base)
+    assert H.rel_err(got2 - base, exp) <= 10 * tol
+
+
+def test_expectation_known_answers_and_invariants(hip):
+    H.say("expectation known answers")
+    a = np.arange(8).astype(np.complex128).reshape(8, 1)
+    np.testing.assert_allclose(H.op_expectation(ops_of(["III", "III"], [1, 1]), a), [280], rtol=1e-15)
+    assert np.all(H.op_expectation(ops_of(["IXYZ", "ZYXI"], [1, 2]), np.zeros((16, 16))) == 0)
+    # sum rule over all 4^n strings for normalised states
+    n = 4
+    strings = ["".join(p) for p in it.product("IXYZ", repeat=n)]
+    psi = syn.random_states(1 << n, 5, seed=5)
+    ev = H.ev_matrix(ops_of(strings), psi)
+    np.testing.assert_allclose((ev ** 2).sum(axis=0), 2 ** n, rtol=1e-13)
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
+def test_strings_expectation_matrix(hip, dtype):
+    n, N, S = 8, 200, 33
+    H.say(f"ev matrix n={n} N={N} S={S} dtype={dtype}")
+    codes = syn.random_codes_max_weight(N, n, 3, seed=2)
+    strings = syn.codes_to_strings(codes)
+    psi = syn.random_states(1 << n, S, seed=2).astype(H.CDT[dtype])
+    got = H.ev_matrix(_abi.Ops(codes), psi, dtype)
+    p128 = psi.astype(np.complex128)
+    exp = np.stack([O.string_expectation(s, 1, p128, np.zeros(S, np.complex128)) for s in strings])
+    tol = 1e-12 if dtype == C128 else 1e-5
+    assert np.abs(exp.imag).max() < 1e-15
+    assert H.rel_err(got, exp.real) <= tol
+
+
+# ---------------------------------------------------------------------------
+# SummedPauliOp::expectation_value and the MFMA contraction
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("mode", [_abi.FP_CONTRACT_FP64, _abi.FP_CONTRACT_FP32, _abi.FP_CONTRACT_BF16X3])
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (16, 16, 4), (200, 1000, 2000), (33, 65, 129)])
+def test_gemm(hip, mode, m, n, k):
+    H.say(f"gemm mode={mode} m={m} n={n} k={k}")
+    a = syn.random_real_matrix(m, k, seed=m)
+    b = syn.random_real_matrix(k, n, seed=n)
+    got = H.gemm(mode, a, b)
+    dt = np.float64 if mode == _abi.FP_CONTRACT_FP64 else np.float32
+    exp = a.astype(dt).astype(np.float64) @ b.astype(dt).astype(np.float64)
+    tol = {_abi.FP_CONTRACT_FP64: 1e-13, _abi.FP_CONTRACT_FP32: 1e-6, _abi.FP_CONTRACT_BF16X3: 3e-5}[mode]
+    assert H.rel_err(got, exp) <= tol
+
+
+@pytest.mark.parametrize("dtype,mode,tol", [
+    (C128, _abi.FP_CONTRACT_FP64, 1e-12),
+    (C128, _abi.FP_CONTRACT_FP32, 1e-5),
+    (C64, _abi.FP_CONTRACT_FP32, 1e-5),
+    (C64, _abi.FP_CONTRACT_BF16X3, 1e-4),
+])
+def test_summed_expectation(hip, dtype, mode, tol):
+    n, N, K, S = 8, 150, 20, 40
+    H.say(f"summed_expectation n={n} N={N} K={K} S={S} dtype={dtype} mode={mode}")
+    codes = syn.random_codes_max_weight(N, n, 3, seed=7)
+    strings = syn.codes_to_strings(codes)
+    W = syn.random_real_matrix(K, N, seed=7)  # (operators x strings), real weights
+    coeffs_nk = W.T.astype(np.complex128)
+    ops = _abi.Ops(codes, matrix=coeffs_nk)
+    psi = syn.random_states(1 << n, S, seed=8).astype(H.CDT[dtype])
+    got = H.summed_expectation(ops, psi, dtype, mode)
+    exp = O.summed_expectation(strings, coeffs_nk, psi.astype(np.complex128),
+                               np.zeros((K, S), np.complex128))
+    assert H.rel_err(got, exp) <= tol
+
+
+def test_summed_expectation_complex_coeffs(hip):
+    n, N, K, S = 5, 40, 6, 7
+    H.say("summed_expectation complex coefficients")
+    codes = syn.random_codes(N, n, seed=17)
+    strings = syn.codes_to_strings(codes)
+    rng = np.random.default_rng(17)
+    coeffs_nk = rng.standard_normal((N, K)) + 1j * rng.standard_normal((N, K))
+    ops = _abi.Ops(codes, matrix=coeffs_nk)
+    psi = syn.random_states(1 << n, S, seed=18)
+    got = H.summed_expectation(ops, psi, C128, _abi.FP_CONTRACT_FP64)
+    exp = O.summed_expectation(strings, coeffs_nk, psi, np.zeros((K, S), np.complex128))
+    assert H.rel_err(got, exp) <= 1e-12
+    got1 = H.summed_expectation(ops, psi[:, :1].copy(), C128, _abi.FP_CONTRACT_FP64)
+    assert H.rel_err(got1[:, 0], exp[:, 0]) <= 1e-12
+
+
+# ---------------------------------------------------------------------------
+# synthetic inputs on the device
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("dtype", [C128, C64])
+def test_fill_states_normalised(hip, dtype):
+    dim, S = 1 << 12, 5
+    H.say(f"fill_states dtype={dtype}")
+    d = _abi.DeviceBuffer(dim * S * np.dtype(H.CDT[dtype]).itemsize)
+    _abi.call("fp_hip_fill_states", dtype, d.ptr, dim, S, syn.gpu_seed(18), 1, None)
+    psi = d.to_array(H.CDT[dtype], (dim, S)).astype(np.complex128)
+    np.testing.assert_allclose(np.linalg.norm(psi, axis=0), 1.0, rtol=1e-12 if dtype == C128 else 1e-6)
+    # the device stream follows the host generator up to libm rounding
+    host = syn.random_states(dim, S, seed=18)
+    assert H.rel_err(psi, host) <= (1e-12 if dtype == C128 else 1e-6)
+
+
+# ---------------------------------------------------------------------------
+# errors
+# ---------------------------------------------------------------------------
+def test_shape_errors(hip):
+    H.say("shape errors")
+    ops = ops_of(["XYZ"])
+    d = _abi.DeviceBuffer(16 * 4)
+    with pytest.raises(ValueError):
+        _abi.call("fp_hip_string_apply_batch", ops.handle, 0, C128, d.ptr, d.ptr + 64, 4, 1, 1.0, 0.0, 0, None)
+    with pytest.raises(ValueError):
+        _abi.call("fp_hip_op_expectation", ops.handle, C128, d.ptr, d.ptr, 8, 0, 0, None)
+    with pytest.raises(ValueError):
+        _abi.call("fp_hip_string_apply_batch", ops.handle, 5, C128, d.ptr, d.ptr, 8, 1, 1.0, 0.0, 0, None)
+    with pytest.raises(ValueError):
+        _abi.Ops(np.array([[0, 4]], np.uint8))
