@@ -74,11 +74,24 @@ def build_python() -> str | None:
     return out
 
 
+def build_cpp_test() -> str | None:
+    """The C++ host API test program (tests/cpp/test_host_api.cpp)."""
+    src = os.path.join(ROOT, "tests", "cpp", "test_host_api.cpp")
+    if not os.path.exists(src):
+        return None
+    out = os.path.join(ROOT, "tests", "cpp", "test_host_api")
+    cxx = os.environ.get("CXX", "g++")
+    _run([cxx, "-O2", "-std=c++20", "-Wall", "-Wextra", f"-I{INCLUDE}", f"-I{CPP_INCLUDE}", "-o", out + ".tmp",
+          src, f"-L{PKG}", "-lfastpauli_hip", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/../../fast-pauli_amd/fast_pauli"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all() -> list[str]:
     built = [build_hip()]
-    py = build_python()
-    if py:
-        built.append(py)
+    for extra in (build_python(), build_cpp_test()):
+        if extra:
+            built.append(extra)
     return built
 
 
@@ -88,5 +101,7 @@ if __name__ == "__main__":
         build_hip()
     elif what == "python":
         build_python()
+    elif what == "cpptest":
+        build_cpp_test()
     else:
         build_all()

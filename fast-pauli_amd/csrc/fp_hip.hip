@@ -120,6 +120,10 @@ struct fp_hip_ops
     double *d_wr64 = nullptr, *d_wi64 = nullptr;
     float *d_wr32 = nullptr, *d_wi32 = nullptr;
     bool matrix_imag_zero = true;
+    // h(j, k) interleaved complex, (n_strings x n_operators), for apply_weighted
+    double *d_h128 = nullptr;
+    float *d_h64 = nullptr;
+    uint32_t *d_ny = nullptr;
 };
 
 namespace
@@ -127,8 +131,9 @@ namespace
 
 void free_device(fp_hip_ops *o)
 {
-    void *ptrs[] = {o->d_x, o->d_z, o->d_meta, o->d_cf128, o->d_cf64, o->d_scale, o->d_kappa_c, o->d_kappa,
-                    o->d_rowsum128, o->d_rowsum64, o->d_wr64, o->d_wi64, o->d_wr32, o->d_wi32};
+    void *ptrs[] = {o->d_x,         o->d_z,         o->d_meta,  o->d_cf128, o->d_cf64,  o->d_scale,
+                    o->d_kappa_c,   o->d_kappa,     o->d_rowsum128, o->d_rowsum64, o->d_wr64, o->d_wi64,
+                    o->d_wr32,      o->d_wi32,      o->d_h128,  o->d_h64,   o->d_ny};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -136,6 +141,9 @@ void free_device(fp_hip_ops *o)
     o->d_meta = nullptr;
     o->d_cf128 = o->d_scale = o->d_kappa_c = o->d_kappa = o->d_rowsum128 = o->d_wr64 = o->d_wi64 = nullptr;
     o->d_cf64 = o->d_rowsum64 = o->d_wr32 = o->d_wi32 = nullptr;
+    o->d_h128 = nullptr;
+    o->d_h64 = nullptr;
+    o->d_ny = nullptr;
 }
 
 // (-i)^k as (re, im)
@@ -867,7 +875,8 @@ fp_status fp_hip_ops_create(fp_hip_ops **out, size_t n_qubits, size_t n_strings,
     }
     fp_status st;
     if ((st = upload(&o->d_x, o->x)) != FP_OK || (st = upload(&o->d_z, o->z)) != FP_OK ||
-        (st = upload(&o->d_meta, meta)) != FP_OK || (st = upload_coeff_tables(o)) != FP_OK)
+        (st = upload(&o->d_ny, o->ny)) != FP_OK || (st = upload(&o->d_meta, meta)) != FP_OK ||
+        (st = upload_coeff_tables(o)) != FP_OK)
     {
         free_device(o);
         delete o;
@@ -945,9 +954,14 @@ fp_status fp_hip_ops_set_matrix(fp_hip_ops *ops, size_t n_operators, double cons
         rowsum_cf32[2 * j + 1] = float(gi);
     }
     ops->matrix_imag_zero = imag_zero;
+    std::vector<double> h128(coeffs_nk, coeffs_nk + 2 * N * K);
+    std::vector<float> h64(2 * N * K);
+    for (size_t i = 0; i < 2 * N * K; ++i)
+        h64[i] = float(coeffs_nk[i]);
     for (void **p : {reinterpret_cast<void **>(&ops->d_wr64), reinterpret_cast<void **>(&ops->d_wi64),
                      reinterpret_cast<void **>(&ops->d_wr32), reinterpret_cast<void **>(&ops->d_wi32),
-                     reinterpret_cast<void **>(&ops->d_rowsum128), reinterpret_cast<void **>(&ops->d_rowsum64)})
+                     reinterpret_cast<void **>(&ops->d_rowsum128), reinterpret_cast<void **>(&ops->d_rowsum64),
+                     reinterpret_cast<void **>(&ops->d_h128), reinterpret_cast<void **>(&ops->d_h64)})
     {
         if (*p)
         {
@@ -957,7 +971,8 @@ fp_status fp_hip_ops_set_matrix(fp_hip_ops *ops, size_t n_operators, double cons
     }
     if ((st = upload(&ops->d_wr64, wr)) != FP_OK || (st = upload(&ops->d_wi64, wi)) != FP_OK ||
         (st = upload(&ops->d_wr32, wr32)) != FP_OK || (st = upload(&ops->d_wi32, wi32)) != FP_OK ||
-        (st = upload(&ops->d_rowsum128, rowsum_cf)) != FP_OK || (st = upload(&ops->d_rowsum64, rowsum_cf32)) != FP_OK)
+        (st = upload(&ops->d_rowsum128, rowsum_cf)) != FP_OK || (st = upload(&ops->d_rowsum64, rowsum_cf32)) != FP_OK ||
+        (st = upload(&ops->d_h128, h128)) != FP_OK || (st = upload(&ops->d_h64, h64)) != FP_OK)
         return st;
     return FP_OK;
 }
@@ -1170,10 +1185,86 @@ fp_status fp_hip_summed_apply(fp_hip_ops const *ops, int dtype, void const *stat
     return FP_OK;
 }
 
-fp_status fp_hip_summed_apply_weighted(fp_hip_ops const *, int, void const *, void const *, void *, size_t, size_t,
-                                       int, void *)
+fp_status fp_hip_summed_apply_weighted(fp_hip_ops const *ops, int dtype, void const *states, void const *data,
+                                       void *out, size_t dim, size_t n_states, int accumulate, void *stream)
 {
-    return fail(FP_ERR_UNSUPPORTED, "SummedPauliOp::apply_weighted is not implemented on the GPU yet");
+    fp_status st = check_shape(ops, dtype, states, out, dim, n_states);
+    if (st != FP_OK)
+        return st;
+    if (ops->n_operators == 0)
+        return fail(FP_ERR_INVALID_ARGUMENT, "SummedPauliOp has no coefficient matrix");
+    if (!data)
+        return fail(FP_ERR_INVALID_ARGUMENT, "data is NULL");
+    if (states == out)
+        return fail(FP_ERR_INVALID_ARGUMENT, "apply cannot run in place (out aliases states)");
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    size_t const N = ops->n_strings, K = ops->n_operators, S = n_states;
+    size_t const esz = dtype == FP_C128 ? sizeof(double) : sizeof(float);
+    void *scratch = nullptr;
+    if ((st = scratch_get(s, 2 * N * S * esz, &scratch)) != FP_OK)
+        return st;
+    uint32_t const wb = uint32_t((N * S + fpk::kBlock - 1) / fpk::kBlock);
+    if (dtype == FP_C128)
+        hipLaunchKernelGGL(fpk::k_weighted_coeffs<double>, dim3(wb), dim3(fpk::kBlock), 0, s, ops->d_h128,
+                           static_cast<double const *>(data), static_cast<double *>(scratch), uint64_t(N),
+                           uint64_t(K), uint64_t(S));
+    else
+        hipLaunchKernelGGL(fpk::k_weighted_coeffs<float>, dim3(wb), dim3(fpk::kBlock), 0, s, ops->d_h64,
+                           static_cast<float const *>(data), static_cast<float *>(scratch), uint64_t(N), uint64_t(K),
+                           uint64_t(S));
+    FP_LAUNCHED();
+    VecGeom const g = vec_geom(dtype, dim, n_states, states, out);
+    bool const acc = accumulate != 0;
+    for (uint64_t v0 = 0; v0 < g.n_vec; v0 += kMaxLaunchBlocks * fpk::kBlock)
+    {
+        uint32_t const blocks =
+            uint32_t(std::min<uint64_t>((g.n_vec - v0 + fpk::kBlock - 1) / fpk::kBlock, kMaxLaunchBlocks));
+        auto launch = [&](auto kernel, auto const *in_p, auto *out_p, auto const *w_p) {
+            hipLaunchKernelGGL(kernel, dim3(blocks), dim3(fpk::kBlock), 0, s, in_p, out_p, v0, g.n_vec, g.shift,
+                               g.vpr, g.inv_vpr, ops->d_x, ops->d_z, ops->d_ny, w_p, uint32_t(N), uint64_t(S));
+        };
+        if (dtype == FP_C128)
+        {
+            using V = fpk::VecOf<double, 1>::type;
+            auto const *ip = static_cast<V const *>(states);
+            auto *op = static_cast<V *>(out);
+            auto const *wp = static_cast<double const *>(scratch);
+            if (g.pow2)
+                acc ? launch(fpk::k_op_apply_weighted<double, 1, true, true>, ip, op, wp)
+                    : launch(fpk::k_op_apply_weighted<double, 1, true, false>, ip, op, wp);
+            else
+                acc ? launch(fpk::k_op_apply_weighted<double, 1, false, true>, ip, op, wp)
+                    : launch(fpk::k_op_apply_weighted<double, 1, false, false>, ip, op, wp);
+        }
+        else if (g.nc == 2)
+        {
+            using V = fpk::VecOf<float, 2>::type;
+            auto const *ip = static_cast<V const *>(states);
+            auto *op = static_cast<V *>(out);
+            auto const *wp = static_cast<float const *>(scratch);
+            if (g.pow2)
+                acc ? launch(fpk::k_op_apply_weighted<float, 2, true, true>, ip, op, wp)
+                    : launch(fpk::k_op_apply_weighted<float, 2, true, false>, ip, op, wp);
+            else
+                acc ? launch(fpk::k_op_apply_weighted<float, 2, false, true>, ip, op, wp)
+                    : launch(fpk::k_op_apply_weighted<float, 2, false, false>, ip, op, wp);
+        }
+        else
+        {
+            using V = fpk::VecOf<float, 1>::type;
+            auto const *ip = static_cast<V const *>(states);
+            auto *op = static_cast<V *>(out);
+            auto const *wp = static_cast<float const *>(scratch);
+            if (g.pow2)
+                acc ? launch(fpk::k_op_apply_weighted<float, 1, true, true>, ip, op, wp)
+                    : launch(fpk::k_op_apply_weighted<float, 1, true, false>, ip, op, wp);
+            else
+                acc ? launch(fpk::k_op_apply_weighted<float, 1, false, true>, ip, op, wp)
+                    : launch(fpk::k_op_apply_weighted<float, 1, false, false>, ip, op, wp);
+        }
+        FP_LAUNCHED();
+    }
+    return FP_OK;
 }
 
 fp_status fp_hip_gemm(int contraction, size_t m, size_t n, size_t k, void const *a, void const *b, void *c,

@@ -652,4 +652,105 @@ __global__ __launch_bounds__(kBlock) void k_pack_complex(In const *__restrict__ 
     }
 }
 
+// ---------------------------------------------------------------------------
+// SummedPauliOp::apply_weighted (fast_pauli/cpp/include/__summed_pauli_op.hpp:378-503)
+//
+// Step 1: w(j,t) = sum_k h(j,k) x(k,t), k in order, product then add (the
+//         reference's sequential loop, no contraction).
+// Step 2: out(l,t) (+)= sum_j (m_j(l) psi(l^x_j, t)) * w(j,t), strings in
+//         order j = 0..N-1; m_j psi is an exact sign flip / re-im swap.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_weighted_coeffs(T const *__restrict__ h, T const *__restrict__ x,
+                                                            T *__restrict__ w, uint64_t n_strings, uint64_t n_ops,
+                                                            uint64_t n_states)
+{
+#pragma clang fp contract(off)
+    uint64_t const i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n_strings * n_states)
+        return;
+    uint64_t const j = i / n_states;
+    uint64_t const t = i - j * n_states;
+    T wr = T(0), wi = T(0);
+    for (uint64_t k = 0; k < n_ops; ++k)
+    {
+        T const d = x[k * n_states + t];
+        T const pr = h[2 * (j * n_ops + k)] * d;
+        T const pi = h[2 * (j * n_ops + k) + 1] * d;
+        wr = wr + pr;
+        wi = wi + pi;
+    }
+    w[2 * i] = wr;
+    w[2 * i + 1] = wi;
+}
+
+template <class T, int NC, bool POW2, bool ACC>
+__global__ __launch_bounds__(kBlock) void k_op_apply_weighted(typename VecOf<T, NC>::type const *__restrict__ in,
+                                                              typename VecOf<T, NC>::type *__restrict__ out,
+                                                              uint64_t v_begin, uint64_t n_vec, uint32_t vpr_shift,
+                                                              uint64_t vpr, double inv_vpr,
+                                                              uint64_t const *__restrict__ xs,
+                                                              uint64_t const *__restrict__ zs,
+                                                              uint32_t const *__restrict__ nys,
+                                                              T const *__restrict__ w, uint32_t n_strings,
+                                                              uint64_t n_states)
+{
+    typedef typename VecOf<T, NC>::type V;
+    uint64_t const v = v_begin + uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (v >= n_vec)
+        return;
+    uint64_t row, col;
+    if constexpr (POW2)
+    {
+        row = v >> vpr_shift;
+        col = v & ((uint64_t(1) << vpr_shift) - 1);
+    }
+    else
+    {
+        row = row_of(v, vpr, inv_vpr);
+        col = v - row * vpr;
+    }
+    V acc = ACC ? out[v] : V(0);
+    for (uint32_t s = 0; s < n_strings; ++s)
+    {
+        uint64_t const x = xs[s];
+        uint64_t const sv = POW2 ? (((row ^ x) << vpr_shift) | col) : ((row ^ x) * vpr + col);
+        V const p = in[sv];
+        uint32_t const sg = parity64(row & zs[s]);
+        uint32_t const ny = nys[s] & 3u;
+        // m = (-i)^{ny} * (-1)^{sg}: (re, im) of m * (pr + i pi) without rounding
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+        {
+            T pr = p[2 * c], pi = p[2 * c + 1];
+            T ar, ai;
+            switch (ny)
+            {
+            case 0:
+                ar = pr, ai = pi;
+                break;
+            case 1:
+                ar = pi, ai = -pr;
+                break;
+            case 2:
+                ar = -pr, ai = -pi;
+                break;
+            default:
+                ar = -pi, ai = pr;
+                break;
+            }
+            if (sg)
+                ar = -ar, ai = -ai;
+            uint64_t const t = col * NC + c;
+            T const wr = w[2 * (uint64_t(s) * n_states + t)];
+            T const wi = w[2 * (uint64_t(s) * n_states + t) + 1];
+            T orr, oi;
+            cmul_exact<T>(ar, ai, wr, wi, orr, oi);
+            acc[2 * c] = add_exact<T>(acc[2 * c], orr);
+            acc[2 * c + 1] = add_exact<T>(acc[2 * c + 1], oi);
+        }
+    }
+    out[v] = acc;
+}
+
 } // namespace fpk
