@@ -46,9 +46,9 @@ def _run(cmd: list[str]) -> None:
 
 
 def build_hip() -> str:
-    src = os.path.join(CSRC, "fp_hip.hip")
+    srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
     tmp = LIB_HIP + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", f"-I{INCLUDE}", f"-I{CSRC}", "-o", tmp, src])
+    _run([HIPCC, *HIP_FLAGS, "-shared", f"-I{INCLUDE}", f"-I{CSRC}", "-o", tmp, *srcs])
     os.replace(tmp, LIB_HIP)
     return LIB_HIP
 

@@ -20,8 +20,9 @@
 
 #include "fp_kernels.hpp"
 #include "fp_mfma.hpp"
+#include "fp_runtime.hpp"
 
-namespace
+namespace fprt
 {
 
 thread_local std::string g_last_error;
@@ -36,24 +37,6 @@ fp_status fail(fp_status code, char const *fmt, ...)
     g_last_error = buf;
     return code;
 }
-
-#define FP_HIP(call)                                                                                                   \
-    do                                                                                                                 \
-    {                                                                                                                  \
-        hipError_t const e_ = (call);                                                                                  \
-        if (e_ != hipSuccess)                                                                                          \
-            return fail(e_ == hipErrorOutOfMemory ? FP_ERR_OUT_OF_MEMORY : FP_ERR_RUNTIME, "%s failed: %s (%s:%d)",    \
-                        #call, hipGetErrorString(e_), __FILE__, __LINE__);                                             \
-    } while (0)
-
-#define FP_LAUNCHED()                                                                                                  \
-    do                                                                                                                 \
-    {                                                                                                                  \
-        hipError_t const e_ = hipGetLastError();                                                                       \
-        if (e_ != hipSuccess)                                                                                          \
-            return fail(FP_ERR_RUNTIME, "kernel launch failed: %s (%s:%d)", hipGetErrorString(e_), __FILE__,          \
-                        __LINE__);                                                                                     \
-    } while (0)
 
 // ---------------------------------------------------------------------------
 // Per-stream scratch: reduction partials etc.  Grown on demand (after the
@@ -91,59 +74,45 @@ fp_status scratch_get(hipStream_t stream, size_t bytes, void **out)
     return FP_OK;
 }
 
-} // namespace
-
-// ---------------------------------------------------------------------------
-// Operator sets
-// ---------------------------------------------------------------------------
-struct fp_hip_ops
+VecGeom vec_geom(int dtype, size_t dim, size_t n_states, void const *a, void const *b)
 {
-    size_t n_qubits = 0;
-    size_t n_strings = 0;
-    size_t n_operators = 0;
-    int device = 0;
-    std::vector<uint64_t> x, z;
-    std::vector<uint32_t> ny;
-    std::vector<double> coeffs;  // complex128, n_strings
-    std::vector<double> matrix;  // complex128, n_strings x n_operators
-    // device copies
-    uint64_t *d_x = nullptr, *d_z = nullptr;
-    fpk::StringMeta *d_meta = nullptr;
-    double *d_cf128 = nullptr;   // c_s * (-i)^{nY_s}, complex128
-    float *d_cf64 = nullptr;     // same, complex64
-    double *d_scale = nullptr;   // c_s * kappa_s (complex, FP64): PauliOp EV
-    double *d_kappa_c = nullptr; // (kappa_s, 0): PauliString EV (coefficient applied after)
-    double *d_kappa = nullptr;   // kappa_s (real): per-string EV matrix
-    double *d_rowsum128 = nullptr; // SummedPauliOp::apply folded coefficients (complex128)
-    float *d_rowsum64 = nullptr;
-    // contraction operands: W^T as (n_operators x n_strings) real and imaginary parts
-    double *d_wr64 = nullptr, *d_wi64 = nullptr;
-    float *d_wr32 = nullptr, *d_wi32 = nullptr;
-    bool matrix_imag_zero = true;
-    // h(j, k) interleaved complex, (n_strings x n_operators), for apply_weighted
-    double *d_h128 = nullptr;
-    float *d_h64 = nullptr;
-    uint32_t *d_ny = nullptr;
-};
+    VecGeom g{};
+    bool const aligned16 = ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & 15u) == 0;
+    g.nc = (dtype == FP_C64 && (n_states % 2 == 0) && aligned16) ? 2 : 1;
+    g.vpr = n_states / g.nc;
+    g.n_vec = uint64_t(dim) * g.vpr;
+    g.pow2 = (g.vpr & (g.vpr - 1)) == 0;
+    g.shift = 0;
+    while ((uint64_t(1) << g.shift) < g.vpr)
+        ++g.shift;
+    g.inv_vpr = 1.0 / double(g.vpr);
+    return g;
+}
+
+} // namespace fprt
+
+using namespace fprt;
+
 
 namespace
 {
 
 void free_device(fp_hip_ops *o)
 {
-    void *ptrs[] = {o->d_x,         o->d_z,         o->d_meta,  o->d_cf128, o->d_cf64,  o->d_scale,
-                    o->d_kappa_c,   o->d_kappa,     o->d_rowsum128, o->d_rowsum64, o->d_wr64, o->d_wi64,
-                    o->d_wr32,      o->d_wi32,      o->d_h128,  o->d_h64,   o->d_ny};
+    void *ptrs[] = {o->d_arena, o->d_rowsum128, o->d_rowsum64, o->d_wr64, o->d_wi64,
+                    o->d_wr32,  o->d_wi32,      o->d_h128,     o->d_h64};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
+    o->d_arena = nullptr;
+    o->arena_bytes = 0;
     o->d_x = o->d_z = nullptr;
     o->d_meta = nullptr;
+    o->d_ny = nullptr;
     o->d_cf128 = o->d_scale = o->d_kappa_c = o->d_kappa = o->d_rowsum128 = o->d_wr64 = o->d_wi64 = nullptr;
     o->d_cf64 = o->d_rowsum64 = o->d_wr32 = o->d_wi32 = nullptr;
     o->d_h128 = nullptr;
     o->d_h64 = nullptr;
-    o->d_ny = nullptr;
 }
 
 // (-i)^k as (re, im)
@@ -200,38 +169,142 @@ template <class T> fp_status upload(T **dptr, std::vector<T> const &h)
     return FP_OK;
 }
 
-fp_status upload_coeff_tables(fp_hip_ops *o)
+// Arena layout: x | z | ny | meta | cf128 | cf64 | scale | kappa_c | kappa,
+// each section 256-byte aligned.
+struct ArenaLayout
 {
+    size_t x, z, ny, meta, cf128, cf64, scale, kappa_c, kappa, total;
+};
+
+ArenaLayout arena_layout(size_t N)
+{
+    auto up = [](size_t v) { return (v + 255) / 256 * 256; };
+    ArenaLayout L{};
+    size_t off = 0;
+    L.x = off, off += up(N * sizeof(uint64_t));
+    L.z = off, off += up(N * sizeof(uint64_t));
+    L.ny = off, off += up(N * sizeof(uint32_t));
+    L.meta = off, off += up(N * sizeof(fpk::StringMeta));
+    L.cf128 = off, off += up(2 * N * sizeof(double));
+    L.cf64 = off, off += up(2 * N * sizeof(float));
+    L.scale = off, off += up(2 * N * sizeof(double));
+    L.kappa_c = off, off += up(2 * N * sizeof(double));
+    L.kappa = off, off += up(N * sizeof(double));
+    L.total = off;
+    return L;
+}
+
+template <class T> T *at(void *base, size_t off)
+{
+    return reinterpret_cast<T *>(static_cast<unsigned char *>(base) + off);
+}
+
+// Coefficient-dependent sections of the host mirror.
+void fill_coeff_tables(fp_hip_ops *o, ArenaLayout const &L)
+{
+    void *h = o->h_arena.data();
     size_t const N = o->n_strings;
-    std::vector<double> cf128(2 * N), scale(2 * N), kappa_c(2 * N), kappa(N);
-    std::vector<float> cf64(2 * N);
     for (size_t s = 0; s < N; ++s)
     {
         double ur, ui;
         minus_i_pow(o->ny[s], ur, ui);
         double const cr = o->coeffs[2 * s], ci = o->coeffs[2 * s + 1];
+        double *cf128 = at<double>(h, L.cf128);
         mul_axis_unit(cr, ci, ur, ui, cf128[2 * s], cf128[2 * s + 1]);
         double fr, fi;
         mul_axis_unit(double(float(cr)), double(float(ci)), ur, ui, fr, fi);
-        cf64[2 * s] = float(fr);
-        cf64[2 * s + 1] = float(fi);
+        at<float>(h, L.cf64)[2 * s] = float(fr);
+        at<float>(h, L.cf64)[2 * s + 1] = float(fi);
         double const k = kappa_of(o->x[s], o->ny[s]);
-        kappa[s] = k;
-        kappa_c[2 * s] = k;
-        kappa_c[2 * s + 1] = 0.0;
-        scale[2 * s] = cr * k;
-        scale[2 * s + 1] = ci * k;
+        at<double>(h, L.kappa)[s] = k;
+        at<double>(h, L.kappa_c)[2 * s] = k;
+        at<double>(h, L.kappa_c)[2 * s + 1] = 0.0;
+        at<double>(h, L.scale)[2 * s] = cr * k;
+        at<double>(h, L.scale)[2 * s + 1] = ci * k;
     }
-    fp_status st;
-    if ((st = upload(&o->d_cf128, cf128)) != FP_OK)
-        return st;
-    if ((st = upload(&o->d_cf64, cf64)) != FP_OK)
-        return st;
-    if ((st = upload(&o->d_scale, scale)) != FP_OK)
-        return st;
-    if ((st = upload(&o->d_kappa_c, kappa_c)) != FP_OK)
-        return st;
-    return upload(&o->d_kappa, kappa);
+}
+
+// (Re)builds the string tables of `o` from letter codes and uploads them on
+// `stream`.  Allocates only when the arena must grow.
+fp_status ops_assign(fp_hip_ops *o, size_t n_qubits, size_t n_strings, uint8_t const *codes, double const *coeffs,
+                     hipStream_t stream, bool sync)
+{
+    if (n_qubits == 0 || n_qubits > 62)
+        return fail(FP_ERR_INVALID_ARGUMENT, "n_qubits must be in 1..62 (got %zu)", n_qubits);
+    if (n_strings == 0 || n_strings > 0xffffffffu)
+        return fail(FP_ERR_INVALID_ARGUMENT, "n_strings must be in 1..2^32-1 (got %zu)", n_strings);
+    if (!codes)
+        return fail(FP_ERR_INVALID_ARGUMENT, "codes is NULL");
+    for (size_t i = 0; i < n_qubits * n_strings; ++i)
+        if (codes[i] > 3)
+            return fail(FP_ERR_INVALID_ARGUMENT, "Pauli code must be 0, 1, 2, or 3 (got %u)", unsigned(codes[i]));
+    ArenaLayout const L = arena_layout(n_strings);
+    if (o->upload_done)
+        FP_HIP(hipEventSynchronize(o->upload_done)); // the previous upload may still read h_arena
+    else
+        FP_HIP(hipEventCreateWithFlags(&o->upload_done, hipEventDisableTiming));
+    o->n_qubits = n_qubits;
+    o->n_strings = n_strings;
+    o->n_operators = 0;
+    o->x.resize(n_strings);
+    o->z.resize(n_strings);
+    o->ny.resize(n_strings);
+    o->coeffs.assign(2 * n_strings, 0.0);
+    o->h_arena.assign(L.total, 0);
+    void *h = o->h_arena.data();
+    for (size_t s = 0; s < n_strings; ++s)
+    {
+        uint64_t x = 0, z = 0;
+        for (size_t i = 0; i < n_qubits; ++i)
+        {
+            uint8_t const c = codes[s * n_qubits + i];
+            uint64_t const bit = uint64_t(1) << (n_qubits - 1 - i);
+            if (c == 1 || c == 2)
+                x |= bit;
+            if (c == 2 || c == 3)
+                z |= bit;
+        }
+        uint32_t const ny = uint32_t(__builtin_popcountll(x & z));
+        o->x[s] = x;
+        o->z[s] = z;
+        o->ny[s] = ny;
+        uint32_t const hb = x ? 63u - uint32_t(__builtin_clzll(x)) : 0u;
+        uint32_t const mode = x == 0 ? fpk::kModeSelf : ((ny & 1u) ? fpk::kModeIm : fpk::kModeRe);
+        at<uint64_t>(h, L.x)[s] = x;
+        at<uint64_t>(h, L.z)[s] = z;
+        at<uint32_t>(h, L.ny)[s] = ny;
+        at<fpk::StringMeta>(h, L.meta)[s] = fpk::StringMeta{x, z, hb, mode};
+        o->coeffs[2 * s] = coeffs ? coeffs[2 * s] : 1.0;
+        o->coeffs[2 * s + 1] = coeffs ? coeffs[2 * s + 1] : 0.0;
+    }
+    fill_coeff_tables(o, L);
+    if (o->arena_bytes < L.total)
+    {
+        if (o->d_arena)
+        {
+            FP_HIP(hipStreamSynchronize(stream));
+            FP_HIP(hipFree(o->d_arena));
+            o->d_arena = nullptr;
+            o->arena_bytes = 0;
+        }
+        FP_HIP(hipMalloc(&o->d_arena, L.total));
+        o->arena_bytes = L.total;
+    }
+    void *d = o->d_arena;
+    o->d_x = at<uint64_t>(d, L.x);
+    o->d_z = at<uint64_t>(d, L.z);
+    o->d_ny = at<uint32_t>(d, L.ny);
+    o->d_meta = at<fpk::StringMeta>(d, L.meta);
+    o->d_cf128 = at<double>(d, L.cf128);
+    o->d_cf64 = at<float>(d, L.cf64);
+    o->d_scale = at<double>(d, L.scale);
+    o->d_kappa_c = at<double>(d, L.kappa_c);
+    o->d_kappa = at<double>(d, L.kappa);
+    FP_HIP(hipMemcpyAsync(o->d_arena, o->h_arena.data(), L.total, hipMemcpyHostToDevice, stream));
+    FP_HIP(hipEventRecord(o->upload_done, stream));
+    if (sync)
+        FP_HIP(hipStreamSynchronize(stream));
+    return FP_OK;
 }
 
 fp_status check_ops(fp_hip_ops const *ops)
@@ -263,35 +336,6 @@ fp_status check_shape(fp_hip_ops const *ops, int dtype, void const *a, void cons
         return fail(FP_ERR_INVALID_ARGUMENT, "device buffers must be aligned to the complex element size (%zu bytes)",
                     size_t(align));
     return FP_OK;
-}
-
-// A launch stays below 2^32 work-items: at most 2^23 blocks of 256 threads.
-constexpr uint64_t kMaxLaunchBlocks = uint64_t(1) << 23;
-
-// Launch geometry of the vector view of a (dim x n_states) batch.
-struct VecGeom
-{
-    int nc;           // complexes per vector
-    uint64_t vpr;     // vectors per row
-    uint64_t n_vec;   // total vectors
-    bool pow2;
-    uint32_t shift;
-    double inv_vpr;
-};
-
-VecGeom vec_geom(int dtype, size_t dim, size_t n_states, void const *a, void const *b)
-{
-    VecGeom g{};
-    bool const aligned16 = ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & 15u) == 0;
-    g.nc = (dtype == FP_C64 && (n_states % 2 == 0) && aligned16) ? 2 : 1;
-    g.vpr = n_states / g.nc;
-    g.n_vec = uint64_t(dim) * g.vpr;
-    g.pow2 = (g.vpr & (g.vpr - 1)) == 0;
-    g.shift = 0;
-    while ((uint64_t(1) << g.shift) < g.vpr)
-        ++g.shift;
-    g.inv_vpr = 1.0 / double(g.vpr);
-    return g;
 }
 
 template <class T, int NC, int U, bool POW2, bool ACC>
@@ -776,7 +820,7 @@ fp_status fp_hip_fill_states(int dtype, void *states, size_t dim, size_t n_state
                            static_cast<float const *>(states), uint64_t(dim), uint64_t(n_states), rows_per_block,
                            partial);
     FP_LAUNCHED();
-    hipLaunchKernelGGL(fpk::k_colnorm_final, dim3(uint32_t((n_states + fpk::kBlock - 1) / fpk::kBlock)),
+    hipLaunchKernelGGL(fpk::k_colnorm_final<double>, dim3(uint32_t((n_states + fpk::kBlock - 1) / fpk::kBlock)),
                        dim3(fpk::kBlock), 0, st, partial, n_parts, uint64_t(n_states), inv_norm);
     FP_LAUNCHED();
     uint32_t const b2 = uint32_t((n_elem + fpk::kBlock - 1) / fpk::kBlock);
@@ -828,62 +872,42 @@ fp_status fp_hip_ops_create(fp_hip_ops **out, size_t n_qubits, size_t n_strings,
     if (!out)
         return fail(FP_ERR_INVALID_ARGUMENT, "ops out-pointer is NULL");
     *out = nullptr;
-    if (n_qubits == 0 || n_qubits > 62)
-        return fail(FP_ERR_INVALID_ARGUMENT, "n_qubits must be in 1..62 (got %zu)", n_qubits);
-    if (n_strings == 0 || n_strings > 0xffffffffu)
-        return fail(FP_ERR_INVALID_ARGUMENT, "n_strings must be in 1..2^32-1 (got %zu)", n_strings);
-    if (!codes)
-        return fail(FP_ERR_INVALID_ARGUMENT, "codes is NULL");
     auto *o = new (std::nothrow) fp_hip_ops();
     if (!o)
         return fail(FP_ERR_OUT_OF_MEMORY, "host allocation failed");
-    o->n_qubits = n_qubits;
-    o->n_strings = n_strings;
     (void)hipGetDevice(&o->device);
-    o->x.resize(n_strings);
-    o->z.resize(n_strings);
-    o->ny.resize(n_strings);
-    o->coeffs.assign(2 * n_strings, 0.0);
-    std::vector<fpk::StringMeta> meta(n_strings);
-    for (size_t s = 0; s < n_strings; ++s)
-    {
-        uint64_t x = 0, z = 0;
-        for (size_t i = 0; i < n_qubits; ++i)
-        {
-            uint8_t const c = codes[s * n_qubits + i];
-            if (c > 3)
-            {
-                delete o;
-                return fail(FP_ERR_INVALID_ARGUMENT, "Pauli code must be 0, 1, 2, or 3 (got %u)", unsigned(c));
-            }
-            uint64_t const bit = uint64_t(1) << (n_qubits - 1 - i);
-            if (c == 1 || c == 2)
-                x |= bit;
-            if (c == 2 || c == 3)
-                z |= bit;
-        }
-        o->x[s] = x;
-        o->z[s] = z;
-        o->ny[s] = uint32_t(__builtin_popcountll(x & z));
-        uint32_t h = 0;
-        if (x)
-            h = 63u - uint32_t(__builtin_clzll(x));
-        uint32_t mode = x == 0 ? fpk::kModeSelf : ((o->ny[s] & 1u) ? fpk::kModeIm : fpk::kModeRe);
-        meta[s] = fpk::StringMeta{x, z, h, mode};
-        o->coeffs[2 * s] = coeffs ? coeffs[2 * s] : 1.0;
-        o->coeffs[2 * s + 1] = coeffs ? coeffs[2 * s + 1] : 0.0;
-    }
-    fp_status st;
-    if ((st = upload(&o->d_x, o->x)) != FP_OK || (st = upload(&o->d_z, o->z)) != FP_OK ||
-        (st = upload(&o->d_ny, o->ny)) != FP_OK || (st = upload(&o->d_meta, meta)) != FP_OK ||
-        (st = upload_coeff_tables(o)) != FP_OK)
+    fp_status const st = ops_assign(o, n_qubits, n_strings, codes, coeffs, nullptr, true);
+    if (st != FP_OK)
     {
         free_device(o);
+        if (o->upload_done)
+            (void)hipEventDestroy(o->upload_done);
         delete o;
         return st;
     }
     *out = o;
     return FP_OK;
+}
+
+fp_status fp_hip_ops_assign(fp_hip_ops *ops, size_t n_qubits, size_t n_strings, uint8_t const *codes,
+                            double const *coeffs, void *stream)
+{
+    fp_status st = check_ops(ops);
+    if (st != FP_OK)
+        return st;
+    // a SummedPauliOp matrix belongs to the previous strings
+    for (void **p : {reinterpret_cast<void **>(&ops->d_wr64), reinterpret_cast<void **>(&ops->d_wi64),
+                     reinterpret_cast<void **>(&ops->d_wr32), reinterpret_cast<void **>(&ops->d_wi32),
+                     reinterpret_cast<void **>(&ops->d_rowsum128), reinterpret_cast<void **>(&ops->d_rowsum64),
+                     reinterpret_cast<void **>(&ops->d_h128), reinterpret_cast<void **>(&ops->d_h64)})
+        if (*p)
+        {
+            FP_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+            FP_HIP(hipFree(*p));
+            *p = nullptr;
+        }
+    ops->matrix.clear();
+    return ops_assign(ops, n_qubits, n_strings, codes, coeffs, static_cast<hipStream_t>(stream), false);
 }
 
 fp_status fp_hip_ops_destroy(fp_hip_ops *ops)
@@ -892,6 +916,8 @@ fp_status fp_hip_ops_destroy(fp_hip_ops *ops)
         return FP_OK;
     (void)hipDeviceSynchronize();
     free_device(ops);
+    if (ops->upload_done)
+        (void)hipEventDestroy(ops->upload_done);
     delete ops;
     return FP_OK;
 }
@@ -905,7 +931,10 @@ fp_status fp_hip_ops_set_coeffs(fp_hip_ops *ops, double const *coeffs)
         return fail(FP_ERR_INVALID_ARGUMENT, "coeffs is NULL");
     (void)hipDeviceSynchronize();
     std::memcpy(ops->coeffs.data(), coeffs, 2 * ops->n_strings * sizeof(double));
-    return upload_coeff_tables(ops);
+    ArenaLayout const L = arena_layout(ops->n_strings);
+    fill_coeff_tables(ops, L);
+    FP_HIP(hipMemcpy(ops->d_arena, ops->h_arena.data(), L.total, hipMemcpyHostToDevice));
+    return FP_OK;
 }
 
 fp_status fp_hip_ops_set_matrix(fp_hip_ops *ops, size_t n_operators, double const *coeffs_nk)

@@ -596,16 +596,17 @@ __global__ __launch_bounds__(kBlock) void k_colnorm_partial(T const *__restrict_
     partial[uint64_t(blockIdx.x) * n_states + t] = acc;
 }
 
-__global__ __launch_bounds__(kBlock) void k_colnorm_final(double const *__restrict__ partial, uint64_t n_parts,
-                                                          uint64_t n_states, double *__restrict__ inv_norm)
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_colnorm_final(R const *__restrict__ partial, uint64_t n_parts,
+                                                          uint64_t n_states, R *__restrict__ inv_norm)
 {
     uint64_t const t = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (t >= n_states)
         return;
-    double acc = 0.0;
+    R acc = 0;
     for (uint64_t i = 0; i < n_parts; ++i)
         acc += partial[i * n_states + t];
-    inv_norm[t] = acc > 0.0 ? 1.0 / sqrt(acc) : 1.0;
+    inv_norm[t] = acc > 0 ? R(1) / sqrt(acc) : R(1);
 }
 
 template <class T>

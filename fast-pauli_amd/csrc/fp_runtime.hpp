@@ -1,0 +1,98 @@
+// fp_runtime.hpp — pieces of the runtime shared by the library's translation
+// units: error reporting, per-stream scratch, the operator-set object and the
+// vector view of a row-major batch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "fast_pauli_hip.h"
+#include "fp_kernels.hpp"
+
+namespace fprt
+{
+
+// Records a printf-style message as the calling thread's last error and returns `code`.
+fp_status fail(fp_status code, char const *fmt, ...) __attribute__((format(printf, 2, 3)));
+
+// Per-stream device scratch of at least `bytes`; the pointer stays valid until
+// the next, larger request on the same stream.
+fp_status scratch_get(hipStream_t stream, size_t bytes, void **out);
+
+// A launch stays below 2^32 work-items: at most 2^23 blocks of 256 threads.
+constexpr uint64_t kMaxLaunchBlocks = uint64_t(1) << 23;
+
+// Launch geometry of the vector view of a (dim x n_states) batch.
+struct VecGeom
+{
+    int nc;         // complexes per vector
+    uint64_t vpr;   // vectors per row
+    uint64_t n_vec; // total vectors
+    bool pow2;
+    uint32_t shift;
+    double inv_vpr;
+};
+
+VecGeom vec_geom(int dtype, size_t dim, size_t n_states, void const *a, void const *b);
+
+} // namespace fprt
+
+#define FP_HIP(call)                                                                                                   \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        hipError_t const e_ = (call);                                                                                  \
+        if (e_ != hipSuccess)                                                                                          \
+            return fprt::fail(e_ == hipErrorOutOfMemory ? FP_ERR_OUT_OF_MEMORY : FP_ERR_RUNTIME, "%s failed: %s (%s:%d)",    \
+                        #call, hipGetErrorString(e_), __FILE__, __LINE__);                                             \
+    } while (0)
+
+#define FP_LAUNCHED()                                                                                                  \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        hipError_t const e_ = hipGetLastError();                                                                       \
+        if (e_ != hipSuccess)                                                                                          \
+            return fprt::fail(FP_ERR_RUNTIME, "kernel launch failed: %s (%s:%d)", hipGetErrorString(e_), __FILE__,          \
+                        __LINE__);                                                                                     \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Operator sets
+// ---------------------------------------------------------------------------
+struct fp_hip_ops
+{
+    size_t n_qubits = 0;
+    size_t n_strings = 0;
+    size_t n_operators = 0;
+    int device = 0;
+    std::vector<uint64_t> x, z;
+    std::vector<uint32_t> ny;
+    std::vector<double> coeffs;  // complex128, n_strings
+    std::vector<double> matrix;  // complex128, n_strings x n_operators
+    // Per-string tables live in one device arena (re-targeted in place by
+    // fp_hip_ops_assign, reallocated only when it must grow) mirrored on the host.
+    void *d_arena = nullptr;
+    size_t arena_bytes = 0;
+    std::vector<unsigned char> h_arena;
+    hipEvent_t upload_done = nullptr; // completes when the last upload of h_arena has been read
+    // device copies (pointers into the arena)
+    uint64_t *d_x = nullptr, *d_z = nullptr;
+    fpk::StringMeta *d_meta = nullptr;
+    double *d_cf128 = nullptr;   // c_s * (-i)^{nY_s}, complex128
+    float *d_cf64 = nullptr;     // same, complex64
+    double *d_scale = nullptr;   // c_s * kappa_s (complex, FP64): PauliOp EV
+    double *d_kappa_c = nullptr; // (kappa_s, 0): PauliString EV (coefficient applied after)
+    double *d_kappa = nullptr;   // kappa_s (real): per-string EV matrix
+    double *d_rowsum128 = nullptr; // SummedPauliOp::apply folded coefficients (complex128)
+    float *d_rowsum64 = nullptr;
+    // contraction operands: W^T as (n_operators x n_strings) real and imaginary parts
+    double *d_wr64 = nullptr, *d_wi64 = nullptr;
+    float *d_wr32 = nullptr, *d_wi32 = nullptr;
+    bool matrix_imag_zero = true;
+    // h(j, k) interleaved complex, (n_strings x n_operators), for apply_weighted
+    double *d_h128 = nullptr;
+    float *d_h64 = nullptr;
+    uint32_t *d_ny = nullptr;
+};

@@ -76,6 +76,7 @@ _SIGS = {
     "fp_hip_convert": [_i, _vp, _i, _vp, _sz, _vp],
     "fp_hip_ops_create": [ctypes.POINTER(_vp), _sz, _sz, _vp, _vp],
     "fp_hip_ops_destroy": [_vp],
+    "fp_hip_ops_assign": [_vp, _sz, _sz, _vp, _vp, _vp],
     "fp_hip_ops_set_coeffs": [_vp, _vp],
     "fp_hip_ops_set_matrix": [_vp, _sz, _vp],
     "fp_hip_ops_info": [_vp, ctypes.POINTER(_sz), ctypes.POINTER(_sz), ctypes.POINTER(_sz)],
@@ -89,7 +90,75 @@ _SIGS = {
     "fp_hip_summed_apply_weighted": [_vp, _i, _vp, _vp, _vp, _sz, _sz, _i, _vp],
     "fp_hip_gemm": [_i, _sz, _sz, _sz, _vp, _vp, _vp, _vp],
 }
+
+
+class BatchSchedule(ctypes.Structure):
+    """fp_hip_batch_schedule"""
+    _fields_ = [("tile_vectors", ctypes.c_uint32), ("ev_tile_work", ctypes.c_uint64),
+                ("blocks_per_cu", ctypes.c_uint32), ("order", ctypes.c_int)]
+
+    def as_dict(self) -> dict:
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+
+_SIGS.update({
+    "fp_hip_batch_create": [ctypes.POINTER(_vp)],
+    "fp_hip_batch_destroy": [_vp],
+    "fp_hip_batch_clear": [_vp],
+    "fp_hip_batch_add": [_vp, _i, _vp, _i, _vp, _vp, _sz, _sz, _i],
+    "fp_hip_batch_set_schedule": [_vp, ctypes.POINTER(BatchSchedule)],
+    "fp_hip_batch_get_schedule": [_vp, ctypes.POINTER(BatchSchedule)],
+    "fp_hip_batch_info": [_vp, ctypes.POINTER(_sz), ctypes.POINTER(_sz), ctypes.POINTER(_sz)],
+    "fp_hip_batch_launch": [_vp, _vp],
+    "fp_hip_batch_set_graph": [_vp, _i],
+    "fp_hip_batch_autotune": [_vp, _vp, _i, ctypes.POINTER(BatchSchedule), ctypes.POINTER(ctypes.c_float)],
+})
+
+
+class ServeConfig(ctypes.Structure):
+    """fp_serve_config"""
+    _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("n_slots", ctypes.c_int),
+                ("max_batch", ctypes.c_int), ("poll_us", ctypes.c_int)]
+
+
+class ServeRequest(ctypes.Structure):
+    """fp_serve_request"""
+    _fields_ = [("kind", ctypes.c_int), ("dtype", ctypes.c_int), ("n_qubits", ctypes.c_size_t),
+                ("n_strings", ctypes.c_size_t), ("codes", ctypes.c_void_p), ("coeffs", ctypes.c_void_p),
+                ("n_states", ctypes.c_size_t), ("state_seed", ctypes.c_uint64), ("states", ctypes.c_void_p),
+                ("keep_output", ctypes.c_int)]
+
+
+class ServeResult(ctypes.Structure):
+    """fp_serve_result"""
+    _fields_ = [("done", ctypes.c_int), ("status", ctypes.c_int), ("latency_s", ctypes.c_double),
+                ("queue_s", ctypes.c_double), ("batch_index", ctypes.c_uint64)]
+
+
+class ServeStats(ctypes.Structure):
+    """fp_serve_stats"""
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("failed", ctypes.c_uint64),
+                ("batches", ctypes.c_uint64), ("batched_requests", ctypes.c_uint64), ("queued", ctypes.c_uint64),
+                ("peak_bytes_in_use", ctypes.c_size_t), ("bytes_allocated", ctypes.c_size_t),
+                ("max_concurrent_batches", ctypes.c_int)]
+
+    def as_dict(self) -> dict:
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+
+_SIGS.update({
+    "fp_hip_server_create": [ctypes.POINTER(_vp), ctypes.POINTER(ServeConfig)],
+    "fp_hip_server_destroy": [_vp],
+    "fp_hip_server_submit": [_vp, ctypes.POINTER(ServeRequest), ctypes.POINTER(_u64)],
+    "fp_hip_server_wait": [_vp, _u64, _d, ctypes.POINTER(ServeResult)],
+    "fp_hip_server_fetch": [_vp, _u64, _vp, _sz],
+    "fp_hip_server_release": [_vp, _u64],
+    "fp_hip_server_stats": [_vp, ctypes.POINTER(ServeStats)],
+})
 _RESTYPES = {"fp_hip_version": ctypes.c_char_p, "fp_hip_last_error": _sz}
+
+FP_BATCH_APPLY, FP_BATCH_EXPECT = 0, 1
+FP_BATCH_ORDER_LARGEST_FIRST, FP_BATCH_ORDER_AS_ADDED = 0, 1
 
 
 def lib() -> ctypes.CDLL:

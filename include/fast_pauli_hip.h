@@ -120,6 +120,12 @@ fp_status fp_hip_convert(int src_dtype, void const *src, int dst_dtype, void *ds
 fp_status fp_hip_ops_create(fp_hip_ops **ops, size_t n_qubits, size_t n_strings, uint8_t const *codes,
                             double const *coeffs);
 fp_status fp_hip_ops_destroy(fp_hip_ops *ops);
+/* Re-target an existing set to new strings and coefficients, uploading on `stream`
+ * (asynchronously when the host tables were not reused by a call still in flight);
+ * allocates only when the set grows.  No reference counterpart: lets a serving loop
+ * reuse operator sets without allocating.  Drops an attached coefficient matrix. */
+fp_status fp_hip_ops_assign(fp_hip_ops *ops, size_t n_qubits, size_t n_strings, uint8_t const *codes,
+                            double const *coeffs, void *stream);
 /* Replace the per-string coefficients (PauliOp::scale, extend, ...). */
 fp_status fp_hip_ops_set_coeffs(fp_hip_ops *ops, double const *coeffs);
 /* Attach SummedPauliOp's coefficient matrix h(j, k): complex128, (n_strings x n_operators) row-major,
@@ -176,6 +182,103 @@ fp_status fp_hip_summed_apply(fp_hip_ops const *ops, int dtype, void const *stat
  * row-major device array (double for FP_C128, float for FP_C64). */
 fp_status fp_hip_summed_apply_weighted(fp_hip_ops const *ops, int dtype, void const *states, void const *data,
                                        void *out, size_t dim, size_t n_states, int accumulate, void *stream);
+
+/* ---- uneven batches: many operators in one persistent, graph-captured launch ----
+ * Entries may differ in qubit count, string count, state count, dtype and kind.
+ * Outputs of one entry must not overlap any other entry's buffers.  The first
+ * launch (or the first after a change) cuts the entries into tiles, orders them,
+ * and captures [counter reset, persistent kernel] into a hipGraph; later launches
+ * replay the graph.  No reference counterpart (the reference runs one operator
+ * per call). */
+typedef struct fp_hip_batch fp_hip_batch;
+enum
+{
+    FP_BATCH_APPLY = 0,  /* PauliOp::apply:  out (+)= sum_s h_s P_s psi */
+    FP_BATCH_EXPECT = 1  /* PauliOp::expectation_value: out[t] (+)= sum_s h_s <psi_t|P_s|psi_t> */
+};
+enum
+{
+    FP_BATCH_ORDER_LARGEST_FIRST = 0,
+    FP_BATCH_ORDER_AS_ADDED = 1
+};
+typedef struct fp_hip_batch_schedule
+{
+    uint32_t tile_vectors;  /* 16-byte vectors per apply tile */
+    uint64_t ev_tile_work;  /* strings x pairs x vectors-per-row per expectation tile */
+    uint32_t blocks_per_cu; /* persistent blocks per compute unit */
+    int order;              /* FP_BATCH_ORDER_* */
+} fp_hip_batch_schedule;
+
+fp_status fp_hip_batch_create(fp_hip_batch **batch);
+fp_status fp_hip_batch_destroy(fp_hip_batch *batch);
+fp_status fp_hip_batch_clear(fp_hip_batch *batch);
+fp_status fp_hip_batch_add(fp_hip_batch *batch, int kind, fp_hip_ops const *ops, int dtype, void const *states,
+                           void *out, size_t dim, size_t n_states, int accumulate);
+fp_status fp_hip_batch_set_schedule(fp_hip_batch *batch, fp_hip_batch_schedule const *schedule);
+fp_status fp_hip_batch_get_schedule(fp_hip_batch const *batch, fp_hip_batch_schedule *schedule);
+fp_status fp_hip_batch_info(fp_hip_batch const *batch, size_t *n_entries, size_t *n_tiles, size_t *n_blocks);
+fp_status fp_hip_batch_launch(fp_hip_batch *batch, void *stream);
+/* enabled = 1 (default): launches replay a captured hipGraph; 0: kernels are enqueued
+ * directly (cheaper when every launch has new entries, as in a serving loop). */
+fp_status fp_hip_batch_set_graph(fp_hip_batch *batch, int enabled);
+/* Times schedule candidates (coordinate descent, `reps` launches each) and keeps the fastest.
+ * Trial launches write the outputs. */
+fp_status fp_hip_batch_autotune(fp_hip_batch *batch, void *stream, int reps, fp_hip_batch_schedule *chosen,
+                                float *best_ms);
+
+/* ---- serving: continuous batching of apply / expectation requests ----------------
+ * A scheduler thread admits queued requests (first fit, arrival order) whose device
+ * memory fits under `hbm_budget_bytes`, and runs each admitted group as one uneven
+ * batch on one of `n_slots` streams; requests arriving meanwhile go into the next free
+ * slot, so batches overlap on the device.  States are device-resident: either the
+ * caller's device pointer (`states`) or created on the device from `state_seed`
+ * (normalised N(0,1) columns).  Latency is measured from submit to completion.
+ * No reference counterpart. */
+typedef struct fp_hip_server fp_hip_server;
+typedef struct fp_serve_config
+{
+    size_t hbm_budget_bytes; /* admission limit for request buffers */
+    int n_slots;             /* concurrent batches (streams), 1..16 */
+    int max_batch;           /* requests per batch */
+    int poll_us;             /* completion polling interval while batches run */
+} fp_serve_config;
+typedef struct fp_serve_request
+{
+    int kind;              /* FP_BATCH_APPLY or FP_BATCH_EXPECT */
+    int dtype;             /* FP_C64 or FP_C128 */
+    size_t n_qubits;
+    size_t n_strings;
+    uint8_t const *codes;  /* n_strings x n_qubits letter codes (copied at submit) */
+    double const *coeffs;  /* complex128 per string, or NULL for ones (copied) */
+    size_t n_states;
+    uint64_t state_seed;   /* states created on the device when `states` is NULL */
+    void const *states;    /* optional caller-owned device states (dim x n_states) */
+    int keep_output;       /* apply: keep the output on the device for fp_hip_server_fetch */
+} fp_serve_request;
+typedef struct fp_serve_result
+{
+    int done;
+    fp_status status;
+    double latency_s; /* submit -> completion */
+    double queue_s;   /* submit -> admission */
+    uint64_t batch_index;
+} fp_serve_result;
+typedef struct fp_serve_stats
+{
+    uint64_t submitted, completed, failed, batches, batched_requests, queued;
+    size_t peak_bytes_in_use, bytes_allocated;
+    int max_concurrent_batches;
+} fp_serve_stats;
+
+fp_status fp_hip_server_create(fp_hip_server **server, fp_serve_config const *config);
+fp_status fp_hip_server_destroy(fp_hip_server *server); /* drains the queue first */
+fp_status fp_hip_server_submit(fp_hip_server *server, fp_serve_request const *request, uint64_t *id);
+/* Waits up to timeout_s (negative: forever); result->done = 0 on timeout. */
+fp_status fp_hip_server_wait(fp_hip_server *server, uint64_t id, double timeout_s, fp_serve_result *result);
+/* Copies an expectation result (n_states complex) or a kept apply output (dim x n_states). */
+fp_status fp_hip_server_fetch(fp_hip_server *server, uint64_t id, void *host_out, size_t bytes);
+fp_status fp_hip_server_release(fp_hip_server *server, uint64_t id);
+fp_status fp_hip_server_stats(fp_hip_server *server, fp_serve_stats *stats);
 
 /* ---- real GEMM on MFMA (stage 2 of SummedPauliOp, exposed for testing) ---------
  * C(MxN) = A(MxK) * B(KxN), all row-major, FP64 for FP_CONTRACT_FP64 else FP32 operands/results. */
