@@ -57,6 +57,13 @@ def parse():
     ap.add_argument("--seed", type=int, default=18)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-c64", action="store_true", help="skip the complex64 leg")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the other workloads (n=10 PauliOp.apply, n=14 SummedPauliOp, uneven batch, serving)")
+    ap.add_argument("--qps", type=float, default=50.0, help="serving: Poisson arrival rate")
+    ap.add_argument("--serve-requests", type=int, default=300, help="serving: number of requests")
+    ap.add_argument("--pmc-pass", action="store_true",
+                    help="only run the profiled kernels once each (a plain copy of the state, apply_batch, "
+                         "expectation_value) for hardware-counter runs")
     return ap.parse_args()
 
 
@@ -117,6 +124,20 @@ def main() -> int:
         ev_ms = [m[1].elapsed_ms(m[2]) for m in marks]
         return total_ms / args.steps, float(np.mean(apply_ms)), float(np.mean(ev_ms))
 
+    if args.pmc_pass:
+        # one launch of each kernel whose HBM bytes the counters measure
+        call("fp_hip_copy", out.ptr, psi.ptr, dim * S * 16, stream)
+        call("fp_hip_string_apply_batch", ops_apply.handle, 0, C128, psi.ptr, out.ptr, dim, S, 1.0, 0.0, 0, stream)
+        call("fp_hip_op_expectation", ops_ev.handle, C128, psi.ptr, ev.ptr, dim, S, 0, stream)
+        _abi.synchronize()
+        log(f"pmc pass done: copy 4 GiB, apply_batch, expectation_value x{N} (n={n}, {S} states, complex128)")
+        return 0
+
+    # achievable copy bandwidth on this device (read + write of one 4 GiB buffer)
+    ms_copy = _time_ms(lambda: call("fp_hip_copy", out.ptr, psi.ptr, dim * S * 16, stream), 5)
+    copy_gbs = 2 * dim * S * 16 / (ms_copy / 1e3) / 1e9
+    log(f"copy kernel: {ms_copy:.3f} ms for 2 x 4 GiB -> {copy_gbs:.0f} GB/s")
+
     log(f"complex128: warmup {args.warmup}, timing {args.steps} steps")
     ms_step, ms_apply, ms_ev = timed(C128, psi, out, ev)
     ev128 = ev.to_array(np.complex128, (S,))
@@ -127,8 +148,7 @@ def main() -> int:
     log(f"complex128: step {ms_step:.3f} ms (apply {ms_apply:.3f} ms, EV {ms_ev:.3f} ms)")
 
     result = {
-        "metric": "strings x states/s (apply_batch 1 string + expectation_value 256 strings) at n=24, "
-                  "16 states; % of HBM peak",
+        "metric": "strings\u00d7states/s & % HBM peak at n=24; p50/p99 latency at fixed QPS",
         "value": value,
         "unit": "strings*states/s",
         "n_gpus": 1,
@@ -151,9 +171,11 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": bytes_ev / (ms_ev / 1e3) / 1e9 / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic("k_op_expect", n, S, N),
             "algorithmic_bytes_per_launch": bytes_ev,
             "avg_launch_ms": ms_ev,
+            "frac_of_measured_copy_bw": bytes_ev / (ms_ev / 1e3) / 1e9 / copy_gbs,
+            "measured_copy_gbs": copy_gbs,
         },
         "roofline_apply_batch": {
             "kernel": "k_string_apply (PauliString::apply_batch)",
@@ -165,6 +187,8 @@ def main() -> int:
             "algorithmic_bytes_per_launch": bytes_apply,
             "avg_launch_ms": ms_apply,
             "strings_x_states_per_s": S / (ms_apply / 1e3),
+            "frac_of_measured_copy_bw": bytes_apply / (ms_apply / 1e3) / 1e9 / copy_gbs,
+            "traffic": measured_traffic("k_string_apply", n, S, 1),
         },
     }
 
@@ -203,9 +227,201 @@ def main() -> int:
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args, psi, codes_apply, codes_ev, coeffs_ev)
+    del psi, out
+
+    if not args.no_extra:
+        result["workloads"] = {
+            "pauli_op_apply_n10": bench_op_apply_n10(args),
+            "summed_expectation_n14": bench_summed_n14(args),
+            "uneven_batch_256": bench_uneven_batch(args),
+            "serving": bench_serving(args),
+        }
+        serving = result["workloads"]["serving"]
+        result["p50_ms"] = serving["p50_ms"]
+        result["p99_ms"] = serving["p99_ms"]
+        result["serving_qps"] = serving["offered_qps"]
 
     print(json.dumps(result), flush=True)
     return 0
+
+
+def measured_traffic(kernel: str, n: int, S: int, n_strings: int):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (profiles/pmc_traffic_n24_s16.json: FETCH_SIZE and WRITE_SIZE, each scaled by
+    the plain copy kernel's reading for a known byte count), for the shape it
+    was measured at; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_n24_s16.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        return None
+    entry = data.get("kernels", {}).get(kernel)
+    if not entry or entry.get("n") != n or entry.get("states") != S or entry.get("strings") != n_strings:
+        return None
+    return entry["bytes_per_launch"]
+
+
+def _time_ms(fn, reps):
+    from fast_pauli import _abi
+
+    fn()
+    _abi.synchronize()
+    a, b = _abi.Event(), _abi.Event()
+    a.record(None)
+    for _ in range(reps):
+        fn()
+    b.record(None)
+    b.synchronize()
+    return a.elapsed_ms(b) / reps
+
+
+def bench_op_apply_n10(args) -> dict:
+    """PauliOp::apply, n=10, 64 strings, 128 states, complex128 (cache-resident, latency bound)."""
+    from fast_pauli import _abi, synthetic as syn
+
+    n, N, S = 10, 64, 128
+    log("workload: PauliOp.apply n=10, 64 strings, 128 states")
+    codes = syn.random_codes(N, n, seed=args.seed)
+    coeffs = syn.random_coeffs(N, seed=args.seed)
+    ops = _abi.Ops(codes, coeffs)
+    psi_h = syn.random_states(1 << n, S, seed=args.seed)
+    d_in = _abi.DeviceBuffer.from_array(psi_h)
+    d_out = _abi.DeviceBuffer(psi_h.nbytes)
+    ms = _time_ms(lambda: _abi.call("fp_hip_op_apply", ops.handle, _abi.FP_C128, d_in.ptr, d_out.ptr, 1 << n, S, 0,
+                                    None), 200)
+    out = {"ms": ms, "strings_x_states_per_s": N * S / (ms / 1e3), "dtype": "c128",
+           "config": "n=10, 64 strings (uniform IXYZ), 128 normalised states"}
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        strings = syn.codes_to_strings(codes)
+        o = np.zeros_like(psi_h)
+        O.op_apply(strings, coeffs, psi_h, o, par=True)
+        reps = 5
+        t = time.perf_counter()
+        for _ in range(reps):
+            o[:] = 0
+            O.op_apply(strings, coeffs, psi_h, o, par=True)
+        t_cpu = (time.perf_counter() - t) / reps
+        got = d_out.to_array(np.complex128, psi_h.shape)
+        ref = O.op_apply(strings, coeffs, psi_h, np.zeros_like(psi_h))
+        out["cpu_par_ms"] = t_cpu * 1e3
+        out["cpu_threads"] = O.max_threads()
+        out["bit_exact_vs_oracle_seq"] = bool(np.array_equal(got, ref))
+    return out
+
+
+def bench_summed_n14(args) -> dict:
+    """SummedPauliOp::expectation_value, n=14, 2000 strings (weight <= 3), 200 operators, 1000 states."""
+    from fast_pauli import _abi, synthetic as syn
+
+    n, N, K, S = 14, 2000, 200, 1000
+    dim = 1 << n
+    log("workload: SummedPauliOp.expectation_value n=14, 2000 strings, 200 operators, 1000 states")
+    codes = syn.random_codes_max_weight(N, n, 3, seed=args.seed)
+    W = syn.random_real_matrix(K, N, seed=args.seed)
+    ops = _abi.Ops(codes, matrix=W.T.astype(np.complex128))
+    d128 = _abi.DeviceBuffer(dim * S * 16)
+    d64 = _abi.DeviceBuffer(dim * S * 8)
+    _abi.call("fp_hip_fill_states", _abi.FP_C128, d128.ptr, dim, S, syn.gpu_seed(args.seed), 1, None)
+    _abi.call("fp_hip_convert", _abi.FP_C128, d128.ptr, _abi.FP_C64, d64.ptr, dim * S, None)
+    out128 = _abi.DeviceBuffer(K * S * 16)
+    out64 = _abi.DeviceBuffer(K * S * 8)
+    ev = _abi.DeviceBuffer(N * S * 8)
+    res = {"config": "n=14, 2000 strings of weight <= 3, W (200 x 2000) ~ N(0,1), 1000 normalised states"}
+    modes = [("c128_fp64", _abi.FP_C128, _abi.FP_CONTRACT_FP64, d128, out128),
+             ("c64_fp32_mfma", _abi.FP_C64, _abi.FP_CONTRACT_FP32, d64, out64),
+             ("c64_bf16x3_mfma", _abi.FP_C64, _abi.FP_CONTRACT_BF16X3, d64, out64)]
+    ref = None
+    for name, dt, mode, src, dst in modes:
+        ms = _time_ms(lambda: _abi.call("fp_hip_summed_expectation", ops.handle, dt, mode, src.ptr, dst.ptr, dim, S,
+                                        0, None), 3)
+        ms1 = _time_ms(lambda: _abi.call("fp_hip_strings_expectation_matrix", ops.handle, dt, src.ptr, ev.ptr, dim,
+                                         S, None), 3)
+        got = dst.to_array(np.complex128 if dt == _abi.FP_C128 else np.complex64, (K, S)).astype(np.complex128)
+        entry = {"ms": ms, "stage1_ms": ms1, "strings_x_states_per_s": N * S / (ms / 1e3)}
+        if ref is None:
+            ref = got
+        else:
+            entry["rel_err_vs_c128_fp64"] = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+        res[name] = entry
+    # the contraction alone on MFMA (200 x 2000) x (2000 x 1000)
+    for name, mode, dt, peak in (("gemm_fp64", _abi.FP_CONTRACT_FP64, np.float64, 78.6),
+                                 ("gemm_fp32", _abi.FP_CONTRACT_FP32, np.float32, 157.3),
+                                 ("gemm_bf16x3", _abi.FP_CONTRACT_BF16X3, np.float32, 2500.0)):
+        A = _abi.DeviceBuffer.from_array(W.astype(dt))
+        B = _abi.DeviceBuffer.from_array(syn.random_real_matrix(N, S, seed=1).astype(dt))
+        C = _abi.DeviceBuffer(K * S * np.dtype(dt).itemsize)
+        ms = _time_ms(lambda: _abi.call("fp_hip_gemm", mode, K, S, N, A.ptr, B.ptr, C.ptr, None), 20)
+        flops = 2.0 * K * N * S * (3 if mode == _abi.FP_CONTRACT_BF16X3 else 1)
+        res[name] = {"ms": ms, "tflops": flops / (ms / 1e3) / 1e12,
+                     "mfma_peak_tflops": peak, "frac_of_peak": flops / (ms / 1e3) / 1e12 / peak}
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        # bounded CPU sample: 200 of the 2000 strings, scaled by 10 (strings run independently per thread)
+        psi = d128.to_array(np.complex128, (dim, S))
+        strings = syn.codes_to_strings(codes[:200])
+        o = np.zeros((K, S), np.complex128)
+        t = time.perf_counter()
+        O.summed_expectation(strings, W.T[:200].astype(np.complex128), psi, o, par=True)
+        t_cpu = (time.perf_counter() - t) * (N / 200)
+        res["cpu_par_ms_estimate"] = t_cpu * 1e3
+        res["cpu_sample"] = "200 of 2000 strings, time x10"
+        res["cpu_threads"] = O.max_threads()
+    return res
+
+
+def bench_uneven_batch(args) -> dict:
+    """256 PauliOps (n 8..18, 16..512 strings, 1..64 states, complex64) in one persistent launch."""
+    from fast_pauli import _abi, synthetic as syn
+    from fast_pauli.batch import Batch, uneven_workload
+
+    log("workload: uneven batch of 256 PauliOp.apply")
+    specs = uneven_workload(256, seed=args.seed)
+    b = Batch()
+    keep = []
+    strst = 0
+    model_bytes = 0
+    for sp in specs:
+        n, N, S = sp["n"], sp["strings"], sp["states"]
+        ops = _abi.Ops(syn.random_codes(N, n, seed=sp["seed"]), syn.random_coeffs(N, seed=sp["seed"]))
+        d_in = _abi.DeviceBuffer((1 << n) * S * 8)
+        d_out = _abi.DeviceBuffer((1 << n) * S * 8)
+        _abi.call("fp_hip_fill_states", _abi.FP_C64, d_in.ptr, 1 << n, S, sp["seed"], 1, None)
+        b.add_apply(ops, d_in.ptr, d_out.ptr, 1 << n, S, _abi.FP_C64)
+        keep.append((ops, d_in, d_out, n, S))
+        strst += N * S
+        model_bytes += (N + 1) * (1 << n) * S * 8
+    tune = b.autotune(reps=2)
+    ms = _time_ms(lambda: b.launch(), 5)
+    d_ref = _abi.DeviceBuffer(max((1 << n) * S * 8 for *_, n, S in keep))
+
+    def one_by_one():
+        for ops, d_in, _, n, S in keep:
+            _abi.call("fp_hip_op_apply", ops.handle, _abi.FP_C64, d_in.ptr, d_ref.ptr, 1 << n, S, 0, None)
+
+    ms_seq = _time_ms(one_by_one, 2)
+    return {"ms": ms, "strings_x_states_per_s": strst / (ms / 1e3), "one_launch_per_op_ms": ms_seq,
+            "model_bytes_per_launch": model_bytes, "model_gbs": model_bytes / (ms / 1e3) / 1e9,
+            "schedule": tune["schedule"], "schedule_cached": tune["cached"], "info": b.info(),
+            "config": "256 PauliOp.apply, n ~ U{8..18}, strings log-uniform 16..512, states ~ U{1..64}, complex64, "
+                      "one persistent hipGraph launch"}
+
+
+def bench_serving(args) -> dict:
+    """Poisson request stream: 70% expectation_value / 30% apply, n 10..22, 8..1024 strings, 1..32 states."""
+    from fast_pauli.serving import Server, request_mix, run_poisson
+
+    log(f"workload: serving, {args.serve_requests} requests at {args.qps} QPS")
+    reqs = request_mix(args.serve_requests, seed=args.seed)
+    with Server(hbm_budget_bytes=200 << 30, n_slots=4, max_batch=32) as srv:
+        out = run_poisson(srv, reqs, qps=args.qps, seed=args.seed)
+    out["config"] = ("Poisson arrivals, 70% PauliOp.expectation_value / 30% PauliOp.apply, n ~ U{10..22}, "
+                     "strings ~ U{8..1024}, states ~ U{1..32}, complex64 device-resident, 200 GB admission budget, "
+                     "4 concurrent batch slots")
+    return out
 
 
 def cpu_baseline(args, psi_dev, codes_apply, codes_ev, coeffs_ev) -> dict:

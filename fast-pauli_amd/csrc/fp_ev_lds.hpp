@@ -1,0 +1,219 @@
+// fp_ev_lds.hpp — per-string expectation values with the state held in LDS.
+//
+// For n <= 14 (complex64) or n <= 13 (complex128) one state column fits in a
+// workgroup's LDS (160 KiB on gfx950).  A block loads column t once (from a
+// (states x rows) transposed copy, so the load is contiguous) and evaluates
+// every string against it; the state is read from HBM once per block instead
+// of once per string.
+//
+// Strings are grouped on the host into chunks of up to kChunk strings sharing
+// the same x mask, so the two LDS reads and the product w = conj(psi_j) psi_{j^x}
+// of a row serve every string of the chunk.  Rows are enumerated as
+// j = u * 64 + lane with u uniform across the wave, which splits the sign
+// (-1)^{popcount(j & z)} into a per-lane bit (low 6 bits of z, fixed for the
+// chunk) and a wave-uniform bit (popcount(u & z >> 6), scalar ALU).
+//   mode kAllRows: every row j contributes eps * sign * G(w_j), G = Re w (nY even)
+//                  or Im w (nY odd), eps = +1 for nY = 0, 1 mod 4, -1 otherwise;
+//                  used when x = 0 or the highest set bit of x is below 6.
+//   mode kPairs:   rows with bit h of j clear contribute 2 eps sign G(w_j) (the
+//                  partner row's term is the same); used when h >= 6, where the
+//                  leader rows are whole 64-row blocks u with bit h-6 clear.
+// Per-lane row sums use RowAcc<T> (FP32 for complex64); the wave reduction and
+// the final scale are done in the same precision and written to ev(s, t).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fp_kernels.hpp"
+
+namespace fpl
+{
+
+constexpr int kChunk = 8;
+constexpr int kThreads = 1024;
+
+enum : uint32_t
+{
+    kAllRows = 0,
+    kPairs = 1
+};
+
+struct EvChunk
+{
+    uint32_t x;     // x mask (n <= 14: fits 32 bits)
+    uint32_t h;     // highest set bit of x (kPairs only)
+    uint32_t mode;  // kAllRows / kPairs
+    uint32_t count; // strings in this chunk (1..kChunk)
+    uint32_t first; // index of the first EvStr
+    uint32_t pad[3];
+};
+
+struct EvStr
+{
+    uint32_t z;
+    uint32_t imag; // 1: G = Im w, 0: G = Re w
+    uint32_t out;  // output row (original string index)
+    uint32_t pad;
+    double scale;  // eps (kAllRows) or 2 eps (kPairs)
+};
+
+__device__ __forceinline__ float flip_sign(float v, uint32_t bits)
+{
+    return __uint_as_float(__float_as_uint(v) ^ bits);
+}
+
+__device__ __forceinline__ double flip_sign(double v, uint32_t bits)
+{
+    return __longlong_as_double(__double_as_longlong(v) ^ (static_cast<long long>(bits) << 32));
+}
+
+template <class T> __device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// (rows x states) row-major -> (states x rows): 32 x 32 tiles through LDS.
+template <class V>
+__global__ __launch_bounds__(256) void k_transpose(V const *__restrict__ in, V *__restrict__ out, uint64_t rows,
+                                                   uint64_t cols)
+{
+    __shared__ V tile[32][33];
+    uint64_t const r0 = uint64_t(blockIdx.y) * 32, c0 = uint64_t(blockIdx.x) * 32;
+    uint32_t const tx = threadIdx.x & 31, ty = threadIdx.x >> 5; // 32 x 8
+    for (uint32_t k = ty; k < 32; k += 8)
+    {
+        uint64_t const r = r0 + k, c = c0 + tx;
+        if (r < rows && c < cols)
+            tile[k][tx] = in[r * cols + c];
+    }
+    __syncthreads();
+    for (uint32_t k = ty; k < 32; k += 8)
+    {
+        uint64_t const c = c0 + k, r = r0 + tx;
+        if (r < rows && c < cols)
+            out[c * rows + r] = tile[tx][k];
+    }
+}
+
+template <class T, int U>
+__device__ __forceinline__ void chunk_rows(typename fpk::VecOf<T, 1>::type const *col, uint32_t lane, EvChunk const &ch,
+                                           uint32_t n_blocks64, uint32_t const (&zhi)[kChunk],
+                                           uint32_t const (&lsign)[kChunk], bool const (&imag)[kChunk],
+                                           typename fpk::RowAcc<T>::type (&acc)[kChunk])
+{
+    typedef typename fpk::RowAcc<T>::type A;
+    uint32_t const x = ch.x;
+    uint32_t const count = ch.count;
+    bool const pairs = ch.mode == kPairs;
+    uint32_t const hh = pairs ? ch.h - 6 : 0;
+    uint32_t const lo_mask = (1u << hh) - 1;
+    uint32_t const n_iter = pairs ? (n_blocks64 >> 1) : n_blocks64;
+    for (uint32_t q0 = 0; q0 < n_iter; q0 += U)
+    {
+        typename fpk::VecOf<T, 1>::type a[U], b[U];
+        uint32_t u[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k)
+        {
+            uint32_t const q = q0 + k < n_iter ? q0 + k : q0;
+            u[k] = pairs ? (((q >> hh) << (hh + 1)) | (q & lo_mask)) : q;
+            uint32_t const j = (u[k] << 6) | lane;
+            a[k] = col[j];
+            b[k] = col[j ^ x];
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k)
+        {
+            if (q0 + k < n_iter)
+            {
+            A const ar = A(a[k][0]), ai = A(a[k][1]), br = A(b[k][0]), bi = A(b[k][1]);
+            A const gre = ar * br + ai * bi;
+            A const gim = ar * bi - ai * br;
+#pragma unroll
+            for (int s = 0; s < kChunk; ++s)
+            {
+                if (s < int(count))
+                {
+                    uint32_t const ub = uint32_t(__popc(u[k] & zhi[s]) & 1) << 31;
+                    A const g = imag[s] ? gim : gre;
+                    acc[s] += flip_sign(g, lsign[s] ^ ub);
+                }
+            }
+            }
+        }
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kThreads) void k_ev_lds(typename fpk::VecOf<T, 1>::type const *__restrict__ states_T,
+                                                     uint32_t dim, EvChunk const *__restrict__ chunks,
+                                                     uint32_t n_chunks, EvStr const *__restrict__ strs,
+                                                     uint64_t n_states, T *__restrict__ ev)
+{
+    typedef typename fpk::VecOf<T, 1>::type V;
+    typedef typename fpk::RowAcc<T>::type A;
+    extern __shared__ __align__(16) unsigned char smem[];
+    V *col = reinterpret_cast<V *>(smem);
+    __shared__ uint32_t next_chunk;
+    uint64_t const t = blockIdx.x;
+    V const *src = states_T + t * dim;
+    for (uint32_t i = threadIdx.x; i < dim; i += kThreads)
+        col[i] = src[i];
+    // chunks [c_begin, c_end) of this block (gridDim.y splits the chunk list)
+    uint32_t const per = (n_chunks + gridDim.y - 1) / gridDim.y;
+    uint32_t const c_begin = blockIdx.y * per;
+    uint32_t const c_end = c_begin + per < n_chunks ? c_begin + per : n_chunks;
+    if (threadIdx.x == 0)
+        next_chunk = c_begin;
+    __syncthreads();
+    uint32_t const lane = threadIdx.x & 63u;
+    uint32_t const n_blocks64 = dim >> 6;
+    while (true)
+    {
+        uint32_t c = 0;
+        if (lane == 0)
+            c = atomicAdd(&next_chunk, 1u);
+        c = __builtin_amdgcn_readfirstlane(__shfl(c, 0, 64));
+        if (c >= c_end)
+            break;
+        EvChunk const ch = chunks[c];
+        uint32_t zhi[kChunk], lsign[kChunk];
+        bool imag[kChunk];
+        A acc[kChunk];
+#pragma unroll
+        for (int s = 0; s < kChunk; ++s)
+        {
+            acc[s] = A(0);
+            zhi[s] = 0;
+            lsign[s] = 0;
+            imag[s] = false;
+            if (s < int(ch.count))
+            {
+                EvStr const st = strs[ch.first + s];
+                zhi[s] = st.z >> 6;
+                lsign[s] = uint32_t(__popc(lane & st.z & 63u) & 1) << 31;
+                imag[s] = st.imag != 0;
+            }
+        }
+        chunk_rows<T, 4>(col, lane, ch, n_blocks64, zhi, lsign, imag, acc);
+#pragma unroll
+        for (int s = 0; s < kChunk; ++s)
+        {
+            if (s < int(ch.count))
+            {
+                A const sum = wave_sum(acc[s]);
+                if (lane == 0)
+                {
+                    EvStr const st = strs[ch.first + s];
+                    ev[uint64_t(st.out) * n_states + t] = T(double(sum) * st.scale);
+                }
+            }
+        }
+    }
+}
+
+} // namespace fpl

@@ -173,7 +173,7 @@ template <class T> fp_status upload(T **dptr, std::vector<T> const &h)
 // each section 256-byte aligned.
 struct ArenaLayout
 {
-    size_t x, z, ny, meta, cf128, cf64, scale, kappa_c, kappa, total;
+    size_t x, z, ny, meta, cf128, cf64, scale, kappa_c, kappa, ev_chunks, ev_strs, total;
 };
 
 ArenaLayout arena_layout(size_t N)
@@ -190,6 +190,8 @@ ArenaLayout arena_layout(size_t N)
     L.scale = off, off += up(2 * N * sizeof(double));
     L.kappa_c = off, off += up(2 * N * sizeof(double));
     L.kappa = off, off += up(N * sizeof(double));
+    L.ev_chunks = off, off += up(N * sizeof(fpl::EvChunk));
+    L.ev_strs = off, off += up(N * sizeof(fpl::EvStr));
     L.total = off;
     return L;
 }
@@ -222,6 +224,63 @@ void fill_coeff_tables(fp_hip_ops *o, ArenaLayout const &L)
         at<double>(h, L.scale)[2 * s] = cr * k;
         at<double>(h, L.scale)[2 * s + 1] = ci * k;
     }
+}
+
+// Groups strings with equal x into chunks of at most fpl::kChunk for the LDS
+// expectation kernel, largest work first.  Returns the number of chunks.
+uint32_t build_ev_chunks(fp_hip_ops const *o, fpl::EvChunk *chunks, fpl::EvStr *strs)
+{
+    size_t const N = o->n_strings;
+    std::vector<uint32_t> order(N);
+    for (size_t i = 0; i < N; ++i)
+        order[i] = uint32_t(i);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return o->x[a] < o->x[b]; });
+    struct Pending
+    {
+        fpl::EvChunk c;
+        std::vector<fpl::EvStr> s;
+        double cost;
+    };
+    std::vector<Pending> pend;
+    size_t i = 0;
+    while (i < N)
+    {
+        size_t j = i;
+        while (j < N && o->x[order[j]] == o->x[order[i]])
+            ++j;
+        uint64_t const x = o->x[order[i]];
+        uint32_t const h = x ? 63u - uint32_t(__builtin_clzll(x)) : 0u;
+        uint32_t const mode = (x != 0 && h >= 6) ? fpl::kPairs : fpl::kAllRows;
+        for (size_t k = i; k < j; k += fpl::kChunk)
+        {
+            Pending p{};
+            p.c.x = uint32_t(x);
+            p.c.h = h;
+            p.c.mode = mode;
+            p.c.count = uint32_t(std::min<size_t>(fpl::kChunk, j - k));
+            for (size_t m = k; m < k + p.c.count; ++m)
+            {
+                uint32_t const sidx = order[m];
+                uint32_t const ny = o->ny[sidx];
+                double const eps = ((ny & 3u) <= 1u) ? 1.0 : -1.0;
+                p.s.push_back(fpl::EvStr{uint32_t(o->z[sidx]), ny & 1u, sidx, 0u,
+                                         eps * (mode == fpl::kPairs ? 2.0 : 1.0)});
+            }
+            p.cost = double(p.c.count + 2) * (mode == fpl::kPairs ? 1.0 : 2.0);
+            pend.push_back(std::move(p));
+        }
+        i = j;
+    }
+    std::stable_sort(pend.begin(), pend.end(), [](Pending const &a, Pending const &b) { return a.cost > b.cost; });
+    uint32_t first = 0;
+    for (size_t c = 0; c < pend.size(); ++c)
+    {
+        pend[c].c.first = first;
+        chunks[c] = pend[c].c;
+        for (auto const &e : pend[c].s)
+            strs[first++] = e;
+    }
+    return uint32_t(pend.size());
 }
 
 // (Re)builds the string tables of `o` from letter codes and uploads them on
@@ -278,6 +337,7 @@ fp_status ops_assign(fp_hip_ops *o, size_t n_qubits, size_t n_strings, uint8_t c
         o->coeffs[2 * s + 1] = coeffs ? coeffs[2 * s + 1] : 0.0;
     }
     fill_coeff_tables(o, L);
+    o->n_ev_chunks = build_ev_chunks(o, at<fpl::EvChunk>(h, L.ev_chunks), at<fpl::EvStr>(h, L.ev_strs));
     if (o->arena_bytes < L.total)
     {
         if (o->d_arena)
@@ -300,6 +360,8 @@ fp_status ops_assign(fp_hip_ops *o, size_t n_qubits, size_t n_strings, uint8_t c
     o->d_scale = at<double>(d, L.scale);
     o->d_kappa_c = at<double>(d, L.kappa_c);
     o->d_kappa = at<double>(d, L.kappa);
+    o->d_ev_chunks = at<fpl::EvChunk>(d, L.ev_chunks);
+    o->d_ev_strs = at<fpl::EvStr>(d, L.ev_strs);
     FP_HIP(hipMemcpyAsync(o->d_arena, o->h_arena.data(), L.total, hipMemcpyHostToDevice, stream));
     FP_HIP(hipEventRecord(o->upload_done, stream));
     if (sync)
@@ -509,12 +571,50 @@ fp_status run_op_expect(fp_hip_ops const *ops, int dtype, void const *states, vo
     return FP_OK;
 }
 
-// Scratch bytes of the stage-1 partials of run_strings_expect.
+// The state column fits in a workgroup's LDS: the x-grouped LDS kernel applies.
+bool lds_eligible(fp_hip_ops const *ops, int dtype, size_t dim)
+{
+    size_t const esz = dtype == FP_C128 ? 16 : 8;
+    return ops->n_qubits >= 6 && dim * esz <= size_t(160 * 1024 - 256) && ops->n_ev_chunks > 0;
+}
+
+// Scratch bytes of stage 1 (run_strings_expect): the transposed states for the
+// LDS kernel, otherwise the streaming kernel's partials.
 size_t strings_expect_partial_bytes(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_states, void const *states)
 {
+    if (lds_eligible(ops, dtype, dim))
+        return dim * n_states * (dtype == FP_C128 ? 16 : 8);
     VecGeom const vg = vec_geom(dtype, dim, n_states, states, states);
     EvGeom const e = ev_geom(vg, dim, ops->n_strings, true, n_states);
     return size_t(e.gx) * ops->n_strings * n_states * sizeof(double);
+}
+
+template <class T>
+fp_status launch_ev_lds(fp_hip_ops const *ops, void const *states, void *ev, size_t dim, size_t n_states,
+                        void *transposed, hipStream_t st)
+{
+    typedef typename fpk::VecOf<T, 1>::type V;
+    dim3 const tg(uint32_t((n_states + 31) / 32), uint32_t((dim + 31) / 32));
+    hipLaunchKernelGGL(fpl::k_transpose<V>, tg, dim3(256), 0, st, static_cast<V const *>(states),
+                       static_cast<V *>(transposed), uint64_t(dim), uint64_t(n_states));
+    FP_LAUNCHED();
+    size_t const lds = dim * sizeof(V);
+    static bool attr_set[2] = {false, false};
+    int const which = sizeof(T) == 8 ? 1 : 0;
+    if (!attr_set[which])
+    {
+        FP_HIP(hipFuncSetAttribute(reinterpret_cast<void const *>(&fpl::k_ev_lds<T>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256));
+        attr_set[which] = true;
+    }
+    uint32_t gy = 1;
+    if (n_states < 1024)
+        gy = uint32_t(std::min<size_t>(ops->n_ev_chunks, (1024 + n_states - 1) / n_states));
+    hipLaunchKernelGGL(fpl::k_ev_lds<T>, dim3(uint32_t(n_states), gy), dim3(fpl::kThreads), lds, st,
+                       static_cast<V const *>(transposed), uint32_t(dim), ops->d_ev_chunks, ops->n_ev_chunks,
+                       ops->d_ev_strs, uint64_t(n_states), static_cast<T *>(ev));
+    FP_LAUNCHED();
+    return FP_OK;
 }
 
 // Real (n_strings x n_states) matrix of per-string expectation values.
@@ -522,17 +622,23 @@ size_t strings_expect_partial_bytes(fp_hip_ops const *ops, int dtype, size_t dim
 fp_status run_strings_expect(fp_hip_ops const *ops, int dtype, void const *states, void *ev, size_t dim,
                              size_t n_states, hipStream_t st, double *partial)
 {
-    VecGeom const vg = vec_geom(dtype, dim, n_states, states, states);
     size_t const N = ops->n_strings;
-    EvGeom const e = ev_geom(vg, dim, N, true, n_states);
     if (!partial)
     {
         void *scratch = nullptr;
-        fp_status s = scratch_get(st, size_t(e.gx) * N * n_states * sizeof(double), &scratch);
+        fp_status s = scratch_get(st, strings_expect_partial_bytes(ops, dtype, dim, n_states, states), &scratch);
         if (s != FP_OK)
             return s;
         partial = static_cast<double *>(scratch);
     }
+    if (lds_eligible(ops, dtype, dim))
+    {
+        if (dtype == FP_C128)
+            return launch_ev_lds<double>(ops, states, ev, dim, n_states, partial, st);
+        return launch_ev_lds<float>(ops, states, ev, dim, n_states, partial, st);
+    }
+    VecGeom const vg = vec_geom(dtype, dim, n_states, states, states);
+    EvGeom const e = ev_geom(vg, dim, N, true, n_states);
     if (dtype == FP_C128)
         launch_strings_expect<double, 1>(vg, e, states, ops->d_meta, uint32_t(N), n_states, partial, st);
     else if (vg.nc == 2)
@@ -831,6 +937,27 @@ fp_status fp_hip_fill_states(int dtype, void *states, size_t dim, size_t n_state
         hipLaunchKernelGGL(fpk::k_scale_columns<float>, dim3(b2), dim3(fpk::kBlock), 0, st,
                            static_cast<float *>(states), n_elem, uint64_t(n_states), inv_norm);
     FP_LAUNCHED();
+    return FP_OK;
+}
+
+fp_status fp_hip_copy(void *dst, void const *src, size_t bytes, void *stream)
+{
+    if (bytes == 0)
+        return FP_OK;
+    if (!dst || !src || (bytes & 15u) || ((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) & 15u))
+        return fail(FP_ERR_INVALID_ARGUMENT, "copy needs non-NULL, 16-byte aligned buffers and a multiple of 16 bytes");
+    hipStream_t const st = static_cast<hipStream_t>(stream);
+    typedef fpk::VecOf<double, 1>::type V;
+    uint64_t const n_vec = bytes / 16;
+    constexpr int U = 4;
+    uint64_t const per_block = uint64_t(fpk::kBlock) * U;
+    for (uint64_t v0 = 0; v0 < n_vec; v0 += kMaxLaunchBlocks * per_block)
+    {
+        uint64_t const blocks = std::min<uint64_t>((n_vec - v0 + per_block - 1) / per_block, kMaxLaunchBlocks);
+        hipLaunchKernelGGL(fpk::k_copy<U>, dim3(uint32_t(blocks)), dim3(fpk::kBlock), 0, st,
+                           static_cast<V const *>(src), static_cast<V *>(dst), v0, n_vec);
+        FP_LAUNCHED();
+    }
     return FP_OK;
 }
 

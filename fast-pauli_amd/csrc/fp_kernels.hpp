@@ -754,4 +754,30 @@ __global__ __launch_bounds__(kBlock) void k_op_apply_weighted(typename VecOf<T, 
     out[v] = acc;
 }
 
+// Plain streaming copy (16-byte vectors, non-temporal): the reference point for
+// achievable HBM bandwidth and for scaling hardware byte counters.
+template <int U>
+__global__ __launch_bounds__(kBlock) void k_copy(VecOf<double, 1>::type const *__restrict__ in,
+                                                 VecOf<double, 1>::type *__restrict__ out, uint64_t v_begin,
+                                                 uint64_t n_vec)
+{
+    typedef VecOf<double, 1>::type V;
+    uint64_t const base = v_begin + uint64_t(blockIdx.x) * (kBlock * U) + threadIdx.x;
+    V r[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+    {
+        uint64_t const v = base + uint64_t(u) * kBlock;
+        if (v < n_vec)
+            r[u] = __builtin_nontemporal_load(in + v);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+    {
+        uint64_t const v = base + uint64_t(u) * kBlock;
+        if (v < n_vec)
+            __builtin_nontemporal_store(r[u], out + v);
+    }
+}
+
 } // namespace fpk

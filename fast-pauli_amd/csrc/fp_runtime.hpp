@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "fast_pauli_hip.h"
+#include "fp_ev_lds.hpp"
 #include "fp_kernels.hpp"
 
 namespace fprt
@@ -85,6 +86,9 @@ struct fp_hip_ops
     double *d_scale = nullptr;   // c_s * kappa_s (complex, FP64): PauliOp EV
     double *d_kappa_c = nullptr; // (kappa_s, 0): PauliString EV (coefficient applied after)
     double *d_kappa = nullptr;   // kappa_s (real): per-string EV matrix
+    fpl::EvChunk *d_ev_chunks = nullptr; // strings grouped by x for the LDS expectation kernel
+    fpl::EvStr *d_ev_strs = nullptr;
+    uint32_t n_ev_chunks = 0;
     double *d_rowsum128 = nullptr; // SummedPauliOp::apply folded coefficients (complex128)
     float *d_rowsum64 = nullptr;
     // contraction operands: W^T as (n_operators x n_strings) real and imaginary parts

@@ -74,6 +74,7 @@ _SIGS = {
     "fp_hip_event_elapsed_ms": [_vp, _vp, ctypes.POINTER(ctypes.c_float)],
     "fp_hip_fill_states": [_i, _vp, _sz, _sz, _u64, _i, _vp],
     "fp_hip_convert": [_i, _vp, _i, _vp, _sz, _vp],
+    "fp_hip_copy": [_vp, _vp, _sz, _vp],
     "fp_hip_ops_create": [ctypes.POINTER(_vp), _sz, _sz, _vp, _vp],
     "fp_hip_ops_destroy": [_vp],
     "fp_hip_ops_assign": [_vp, _sz, _sz, _vp, _vp, _vp],

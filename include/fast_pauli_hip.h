@@ -107,6 +107,10 @@ fp_status fp_hip_event_elapsed_ms(void *start, void *stop, float *ms);
 fp_status fp_hip_fill_states(int dtype, void *states, size_t dim, size_t n_states, uint64_t seed, int normalise,
                              void *stream);
 
+/* Plain device-to-device copy with a streaming kernel (16-byte aligned, multiple of 16 bytes):
+ * the achievable-bandwidth reference for the roofline and for scaling byte counters. */
+fp_status fp_hip_copy(void *dst, void const *src, size_t bytes, void *stream);
+
 /* Element-wise conversion between complex128 and complex64 batches on the device
  * (round to nearest when narrowing).  n_elements complex values. */
 fp_status fp_hip_convert(int src_dtype, void const *src, int dst_dtype, void *dst, size_t n_elements, void *stream);

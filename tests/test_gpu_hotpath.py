@@ -272,3 +272,30 @@ def test_shape_errors(hip):
         _abi.call("fp_hip_string_apply_batch", ops.handle, 5, C128, d.ptr, d.ptr, 8, 1, 1.0, 0.0, 0, None)
     with pytest.raises(ValueError):
         _abi.Ops(np.array([[0, 4]], np.uint8))
+
+
+@pytest.mark.parametrize("dtype,n,S", [(C64, 6, 3), (C64, 13, 40), (C64, 14, 17), (C128, 13, 5),
+                                       (C128, 9, 1100), (C64, 15, 4)])
+def test_strings_expectation_matrix_all_row_modes(hip, dtype, n, S):
+    """Strings with x = 0, with all X/Y letters on the low 6 row bits, and with
+    higher X/Y letters (the three row enumerations of the LDS kernel; n = 15
+    takes the streaming kernel)."""
+    H.say(f"ev matrix modes n={n} S={S} dtype={dtype}")
+    rng = np.random.default_rng(n * 100 + S)
+    strings = []
+    for _ in range(40):  # diagonal strings
+        strings.append("".join(rng.choice(list("IZ"), n)))
+    for _ in range(40):  # X/Y only on the last 6 characters (row bits 0..5)
+        strings.append("".join(rng.choice(list("IZ"), n - 6)) + "".join(rng.choice(list("IXYZ"), 6)))
+    for _ in range(80):  # anything
+        strings.append("".join(rng.choice(list("IXYZ"), n)))
+    strings += ["X" + "I" * (n - 1), "Y" * n, "I" * n]
+    codes = O.codes_of(strings)
+    psi = syn.random_states(1 << n, S, seed=n + S).astype(H.CDT[dtype])
+    got = H.ev_matrix(_abi.Ops(codes), psi, dtype)
+    p128 = psi.astype(np.complex128)
+    exp = np.stack([O.string_expectation(s, 1, p128, np.zeros(S, np.complex128)).real for s in strings])
+    tol = 1e-12 if dtype == C128 else 1e-5
+    assert H.rel_err(got, exp) <= tol
+    # each row against its own scale: |<P>| <= 1 for normalised states
+    assert np.abs(got - exp).max() <= (1e-13 if dtype == C128 else 2e-6)
