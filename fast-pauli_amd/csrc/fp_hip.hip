@@ -99,6 +99,7 @@ namespace
 
 void free_device(fp_hip_ops *o)
 {
+    fprt::release_tiles(o);
     void *ptrs[] = {o->d_arena, o->d_rowsum128, o->d_rowsum64, o->d_wr64, o->d_wi64,
                     o->d_wr32,  o->d_wi32,      o->d_h128,     o->d_h64};
     for (void *p : ptrs)
@@ -297,6 +298,7 @@ fp_status ops_assign(fp_hip_ops *o, size_t n_qubits, size_t n_strings, uint8_t c
     for (size_t i = 0; i < n_qubits * n_strings; ++i)
         if (codes[i] > 3)
             return fail(FP_ERR_INVALID_ARGUMENT, "Pauli code must be 0, 1, 2, or 3 (got %u)", unsigned(codes[i]));
+    fprt::release_tiles(o); // string groups belong to the previous strings
     ArenaLayout const L = arena_layout(n_strings);
     if (o->upload_done)
         FP_HIP(hipEventSynchronize(o->upload_done)); // the previous upload may still read h_arena
@@ -1222,6 +1224,12 @@ fp_status fp_hip_op_expectation(fp_hip_ops const *ops, int dtype, void const *st
     fp_status st = check_shape(ops, dtype, states, out, dim, n_states);
     if (st != FP_OK)
         return st;
+    bool tiled = false;
+    if ((st = fprt::choose_op_expect_tiled(ops, dtype, n_states, &tiled)) != FP_OK)
+        return st;
+    if (tiled)
+        return fprt::run_op_expect_tiled(ops, dtype, states, out, dim, n_states, accumulate,
+                                         static_cast<hipStream_t>(stream));
     return run_op_expect(ops, dtype, states, out, dim, n_states, 0, ops->n_strings, ops->d_scale, 1.0, 0.0,
                          accumulate, static_cast<hipStream_t>(stream));
 }
@@ -1257,15 +1265,17 @@ fp_status fp_hip_summed_expectation(fp_hip_ops const *ops, int dtype, int contra
     size_t const ev_nat_bytes = round256(N * S * nat_esz);
     size_t const ev_conv_bytes = convert ? round256(N * S * esz) : 0;
     size_t const c_bytes = round256(2 * K * S * esz);
-    size_t const part_bytes = strings_expect_partial_bytes(ops, dtype, dim, S, states);
+    size_t const part_bytes = round256(strings_expect_partial_bytes(ops, dtype, dim, S, states));
+    size_t const split_bytes = fpm::gemm_scratch_bytes(contraction, K, S, N);
     void *scratch = nullptr;
-    if ((st = scratch_get(s, ev_nat_bytes + ev_conv_bytes + c_bytes + part_bytes, &scratch)) != FP_OK)
+    if ((st = scratch_get(s, ev_nat_bytes + ev_conv_bytes + c_bytes + part_bytes + split_bytes, &scratch)) != FP_OK)
         return st;
     char *base = static_cast<char *>(scratch);
     void *ev_nat = base;
     void *ev = convert ? static_cast<void *>(base + ev_nat_bytes) : ev_nat;
     void *cmat = base + ev_nat_bytes + ev_conv_bytes;
     auto *partial = reinterpret_cast<double *>(base + ev_nat_bytes + ev_conv_bytes + c_bytes);
+    void *split = split_bytes ? base + ev_nat_bytes + ev_conv_bytes + c_bytes + part_bytes : nullptr;
     if ((st = run_strings_expect(ops, dtype, states, ev_nat, dim, S, s, partial)) != FP_OK)
         return st;
     if (convert)
@@ -1286,7 +1296,7 @@ fp_status fp_hip_summed_expectation(fp_hip_ops const *ops, int dtype, int contra
         void const *w = fp64 ? static_cast<void const *>(p == 0 ? ops->d_wr64 : ops->d_wi64)
                              : static_cast<void const *>(p == 0 ? ops->d_wr32 : ops->d_wi32);
         void *c = static_cast<char *>(cmat) + p * K * S * esz;
-        if ((st = fpm::gemm(contraction, K, S, N, w, ev, c, s)) != FP_OK)
+        if ((st = fpm::gemm(contraction, K, S, N, w, ev, c, split, s)) != FP_OK)
             return fail(st, "contraction failed: %s", fpm::last_error());
     }
     uint64_t const total = uint64_t(K) * S;
@@ -1428,7 +1438,13 @@ fp_status fp_hip_gemm(int contraction, size_t m, size_t n, size_t k, void const 
 {
     if (!a || !b || !c || m == 0 || n == 0 || k == 0)
         return fail(FP_ERR_INVALID_ARGUMENT, "empty or NULL GEMM operand");
-    fp_status const st = fpm::gemm(contraction, m, n, k, a, b, c, static_cast<hipStream_t>(stream));
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    size_t const split_bytes = fpm::gemm_scratch_bytes(contraction, m, n, k);
+    void *split = nullptr;
+    fp_status st = FP_OK;
+    if (split_bytes && (st = scratch_get(s, split_bytes, &split)) != FP_OK)
+        return st;
+    st = fpm::gemm(contraction, m, n, k, a, b, c, split, s);
     if (st != FP_OK)
         return fail(st, "%s", fpm::last_error());
     return FP_OK;

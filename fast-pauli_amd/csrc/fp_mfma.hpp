@@ -7,16 +7,20 @@
 //   FP64   v_mfma_f64_16x16x4_f64          (plain / oracle mode)
 //   FP32   v_mfma_f32_16x16x4_f32
 //   BF16x3 v_mfma_f32_16x16x16_bf16: each FP32 operand is split into
-//          hi = bf16(x), lo = bf16(x - hi); C = hi*hi + hi*lo + lo*hi with FP32
+//          hi = bf16(x), lo = bf16(x - hi); C = lo*hi + hi*lo + hi*hi with FP32
 //          accumulation (the dropped lo*lo term is ~2^-16 relative).
 //
-// Tiling: a 256-thread block computes a 32 x 32 tile of C as 2 x 2 waves of
-// 16 x 16 MFMA tiles; BK-deep slabs of A and B are staged through LDS with
-// coalesced loads (zero-filled at the edges).  Operand fragment layout for
-// the 16x16xK instructions with one block: lane l holds A[l % 16][k0 + l/16]
-// (and B[k0 + l/16][l % 16]); the f32 accumulator holds C[4*(l/16) + r][l % 16]
-// in register r, the f64 one C[l/16 + 4*r][l % 16].  For the bf16 instruction
-// each lane holds 4 consecutive k.
+// Tiling: a 256-thread block computes a 64 x 64 tile of C; its 4 waves (2 x 2)
+// each own 32 x 32 as 2 x 2 MFMA tiles of 16 x 16.  BK-deep slabs of A and B
+// go through double-buffered LDS: the next slab is loaded into registers while
+// the current one feeds the MFMAs.  The K range is split across gridDim.z when
+// the tile grid alone would not fill the chip; split partials are summed in
+// fixed order by a second kernel, so results are deterministic.
+//
+// Operand layouts of the 16x16xK single-block instructions: lane l holds
+// A[l % 16][k0 + l/16] and B[k0 + l/16][l % 16] (the bf16 form: 4 consecutive
+// k per lane, k0 + 4*(l/16) + e); the f32 accumulator holds C[4*(l/16) + r][l % 16]
+// in register r, the f64 one C[l/16 + 4*r][l % 16].
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -37,8 +41,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef short s16x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kTile = 32; // block tile (M and N)
-constexpr int kPad = 1;
+constexpr int kBM = 64, kBN = 64;
 
 __device__ __forceinline__ uint16_t bf16_rne(float x)
 {
@@ -52,151 +55,255 @@ __device__ __forceinline__ float bf16_to_f32(uint16_t h)
     return __uint_as_float(uint32_t(h) << 16);
 }
 
-template <class T>
-__device__ __forceinline__ T load_or_zero(T const *__restrict__ p, uint64_t r, uint64_t c, uint64_t rows,
-                                          uint64_t cols)
+enum Mode
 {
-    return (r < rows && c < cols) ? p[r * cols + c] : T(0);
-}
+    kF64 = 0,
+    kF32 = 1,
+    kBF16x3 = 2
+};
 
-// FP64 / FP32: K-step 4 per instruction.
-template <class T, int BK>
-__global__ __launch_bounds__(256) void k_gemm_mfma(T const *__restrict__ A, T const *__restrict__ B, T *__restrict__ C,
-                                                   uint64_t M, uint64_t N, uint64_t K)
+template <int MODE> struct Traits;
+template <> struct Traits<kF64>
 {
-    __shared__ T As[kTile][BK + kPad];
-    __shared__ T Bs[BK][kTile + kPad];
+    typedef double T;
+    typedef f64x4 Acc;
+    static constexpr int BK = 16;
+};
+template <> struct Traits<kF32>
+{
+    typedef float T;
+    typedef f32x4 Acc;
+    static constexpr int BK = 16;
+};
+template <> struct Traits<kBF16x3>
+{
+    typedef float T;
+    typedef f32x4 Acc;
+    static constexpr int BK = 32;
+};
+
+// LDS slab: A as [BM][BK] (K contiguous), B as [BN][BK] (K contiguous, i.e.
+// transposed) so every lane reads its k-run contiguously.  BF16x3 stores hi and
+// lo planes of both.
+template <int MODE> struct Slab
+{
+    typedef typename Traits<MODE>::T T;
+    static constexpr int BK = Traits<MODE>::BK;
+    T a[kBM][BK + 1];
+    T b[kBN][BK + 1];
+};
+template <> struct __align__(16) Slab<kBF16x3>
+{
+    static constexpr int BK = Traits<kBF16x3>::BK;
+    uint16_t ah[kBM][BK + 4], al[kBM][BK + 4];
+    uint16_t bh[kBN][BK + 4], bl[kBN][BK + 4];
+};
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_gemm(typename Traits<MODE>::T const *__restrict__ A,
+                                              typename Traits<MODE>::T const *__restrict__ B,
+                                              typename Traits<MODE>::T *__restrict__ C, uint64_t M, uint64_t N,
+                                              uint64_t K, uint64_t k_per_split)
+{
+    typedef typename Traits<MODE>::T T;
+    typedef typename Traits<MODE>::Acc Acc;
+    constexpr int BK = Traits<MODE>::BK;
+    constexpr int A_PER = kBM * BK / 256; // elements of A (and of B) per thread per slab
+    __shared__ Slab<MODE> slab[2];
+
     uint32_t const tid = threadIdx.x;
     uint32_t const lane = tid & 63u;
     uint32_t const wave = tid >> 6;
-    uint32_t const wm = (wave >> 1) * 16, wn = (wave & 1) * 16;
-    uint64_t const m0 = uint64_t(blockIdx.y) * kTile, n0 = uint64_t(blockIdx.x) * kTile;
-    typedef typename std::conditional<sizeof(T) == 8, f64x4, f32x4>::type Acc;
-    Acc acc = {0, 0, 0, 0};
-    for (uint64_t k0 = 0; k0 < K; k0 += BK)
-    {
-        for (uint32_t i = tid; i < kTile * BK; i += 256)
-        {
-            uint32_t const r = i / BK, c = i % BK;
-            As[r][c] = load_or_zero(A, m0 + r, k0 + c, M, K);
-        }
-        for (uint32_t i = tid; i < BK * kTile; i += 256)
-        {
-            uint32_t const r = i / kTile, c = i % kTile;
-            Bs[r][c] = load_or_zero(B, k0 + r, n0 + c, K, N);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < BK; kk += 4)
-        {
-            T const a = As[wm + (lane & 15u)][kk + (lane >> 4)];
-            T const b = Bs[kk + (lane >> 4)][wn + (lane & 15u)];
-            if constexpr (sizeof(T) == 8)
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-            else
-                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-    {
-        // f32: register r holds row 4*(lane/16) + r; f64: row lane/16 + 4*r
-        uint64_t const row = m0 + wm + (sizeof(T) == 8 ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r);
-        uint64_t const col = n0 + wn + (lane & 15u);
-        if (row < M && col < N)
-            C[row * N + col] = acc[r];
-    }
-}
+    uint32_t const wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    uint64_t const m0 = uint64_t(blockIdx.y) * kBM, n0 = uint64_t(blockIdx.x) * kBN;
+    uint64_t const kb = uint64_t(blockIdx.z) * k_per_split;
+    uint64_t const ke = kb + k_per_split < K ? kb + k_per_split : K;
 
-// BF16 split precision, K-step 16 per instruction.
-template <int BK>
-__global__ __launch_bounds__(256) void k_gemm_bf16x3(float const *__restrict__ A, float const *__restrict__ B,
-                                                     float *__restrict__ C, uint64_t M, uint64_t N, uint64_t K)
-{
-    // hi / lo planes; K contiguous per row for A, per column for B (transposed
-    // in LDS so each lane reads 4 consecutive k as one 8-byte load)
-    __shared__ uint16_t Ah[kTile][BK + 4], Al[kTile][BK + 4];
-    __shared__ uint16_t Bh[kTile][BK + 4], Bl[kTile][BK + 4];
-    uint32_t const tid = threadIdx.x;
-    uint32_t const lane = tid & 63u;
-    uint32_t const wave = tid >> 6;
-    uint32_t const wm = (wave >> 1) * 16, wn = (wave & 1) * 16;
-    uint64_t const m0 = uint64_t(blockIdx.y) * kTile, n0 = uint64_t(blockIdx.x) * kTile;
-    f32x4 acc = {0, 0, 0, 0};
-    for (uint64_t k0 = 0; k0 < K; k0 += BK)
-    {
-        for (uint32_t i = tid; i < kTile * BK; i += 256)
+    Acc acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            acc[i][j] = Acc{0, 0, 0, 0};
+
+    // per-thread global -> register prefetch of one slab
+    T ra[A_PER], rb[A_PER];
+    auto fetch = [&](uint64_t k0) {
+#pragma unroll
+        for (int e = 0; e < A_PER; ++e)
         {
+            uint32_t const i = tid + 256u * e;
+            uint32_t const r = i / BK, c = i % BK; // A: row r (M), column c (K)
+            uint64_t const gm = m0 + r, gk = k0 + c;
+            ra[e] = (gm < M && gk < ke) ? A[gm * K + gk] : T(0);
+            uint32_t const br = i / kBN, bc = i % kBN; // B: row br (K), column bc (N)
+            uint64_t const gk2 = k0 + br, gn = n0 + bc;
+            rb[e] = (gk2 < ke && gn < N) ? B[gk2 * N + gn] : T(0);
+        }
+    };
+    auto stage = [&](Slab<MODE> &s) {
+#pragma unroll
+        for (int e = 0; e < A_PER; ++e)
+        {
+            uint32_t const i = tid + 256u * e;
             uint32_t const r = i / BK, c = i % BK;
-            float const v = load_or_zero(A, m0 + r, k0 + c, M, K);
-            uint16_t const h = bf16_rne(v);
-            Ah[r][c] = h;
-            Al[r][c] = bf16_rne(v - bf16_to_f32(h));
-        }
-        for (uint32_t i = tid; i < BK * kTile; i += 256)
-        {
-            uint32_t const r = i / kTile, c = i % kTile; // r = k, c = n
-            float const v = load_or_zero(B, k0 + r, n0 + c, K, N);
-            uint16_t const h = bf16_rne(v);
-            Bh[c][r] = h;
-            Bl[c][r] = bf16_rne(v - bf16_to_f32(h));
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < BK; kk += 16)
-        {
-            uint32_t const ar = wm + (lane & 15u), bc = wn + (lane & 15u);
-            uint32_t const kb = kk + 4 * (lane >> 4);
-            s16x4 ah, al, bh, bl;
-#pragma unroll
-            for (int e = 0; e < 4; ++e)
+            uint32_t const br = i / kBN, bc = i % kBN;
+            if constexpr (MODE == kBF16x3)
             {
-                ah[e] = short(Ah[ar][kb + e]);
-                al[e] = short(Al[ar][kb + e]);
-                bh[e] = short(Bh[bc][kb + e]);
-                bl[e] = short(Bl[bc][kb + e]);
+                uint16_t const h = bf16_rne(ra[e]);
+                s.ah[r][c] = h;
+                s.al[r][c] = bf16_rne(ra[e] - bf16_to_f32(h));
+                uint16_t const g = bf16_rne(rb[e]);
+                s.bh[bc][br] = g;
+                s.bl[bc][br] = bf16_rne(rb[e] - bf16_to_f32(g));
             }
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc, 0, 0, 0);
+            else
+            {
+                s.a[r][c] = ra[e];
+                s.b[bc][br] = rb[e];
+            }
         }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
+    };
+
+    int buf = 0;
+    if (kb < ke)
     {
-        uint64_t const row = m0 + wm + 4 * (lane >> 4) + r;
-        uint64_t const col = n0 + wn + (lane & 15u);
-        if (row < M && col < N)
-            C[row * N + col] = acc[r];
+        fetch(kb);
+        stage(slab[0]);
     }
+    __syncthreads();
+    for (uint64_t k0 = kb; k0 < ke; k0 += BK)
+    {
+        bool const more = k0 + BK < ke;
+        if (more)
+            fetch(k0 + BK);
+        Slab<MODE> const &s = slab[buf];
+        if constexpr (MODE == kBF16x3)
+        {
+#pragma unroll
+            for (int kk = 0; kk < BK; kk += 16)
+            {
+                uint32_t const kq = kk + 4 * (lane >> 4);
+                s16x4 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+                {
+                    uint32_t const ar = wm + 16 * i + (lane & 15u);
+                    uint32_t const bc = wn + 16 * i + (lane & 15u);
+                    // rows are 72 bytes and kq is a multiple of 4: 8-byte aligned
+                    ah[i] = *reinterpret_cast<s16x4 const *>(&s.ah[ar][kq]);
+                    al[i] = *reinterpret_cast<s16x4 const *>(&s.al[ar][kq]);
+                    bh[i] = *reinterpret_cast<s16x4 const *>(&s.bh[bc][kq]);
+                    bl[i] = *reinterpret_cast<s16x4 const *>(&s.bl[bc][kq]);
+                }
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+                    {
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[i], bh[j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bh[j], acc[i][j], 0, 0, 0);
+                    }
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int kk = 0; kk < BK; kk += 4)
+            {
+                T a[2], b[2];
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+                {
+                    a[i] = s.a[wm + 16 * i + (lane & 15u)][kk + (lane >> 4)];
+                    b[i] = s.b[wn + 16 * i + (lane & 15u)][kk + (lane >> 4)];
+                }
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+                    {
+                        if constexpr (MODE == kF64)
+                            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+                        else
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
+                    }
+            }
+        }
+        if (more)
+            stage(slab[buf ^ 1]);
+        __syncthreads();
+        buf ^= 1;
+    }
+    // C (or this split's partial plane)
+    T *out = C + uint64_t(blockIdx.z) * M * N;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+            {
+                uint32_t const rr = MODE == kF64 ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r;
+                uint64_t const row = m0 + wm + 16 * i + rr;
+                uint64_t const col = n0 + wn + 16 * j + (lane & 15u);
+                if (row < M && col < N)
+                    out[row * N + col] = acc[i][j][r];
+            }
 }
 
-inline fp_status gemm(int mode, size_t m, size_t n, size_t k, void const *a, void const *b, void *c,
+template <class T>
+__global__ __launch_bounds__(256) void k_sum_splits(T const *__restrict__ parts, T *__restrict__ C, uint64_t mn,
+                                                    uint32_t splits)
+{
+    uint64_t const i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= mn)
+        return;
+    T s = parts[i];
+    for (uint32_t p = 1; p < splits; ++p)
+        s += parts[uint64_t(p) * mn + i];
+    C[i] = s;
+}
+
+// Splits of K that bring the block count near the CU count.
+inline uint32_t choose_splits(size_t m, size_t n, size_t k, int bk)
+{
+    size_t const tiles = ((m + kBM - 1) / kBM) * ((n + kBN - 1) / kBN);
+    size_t splits = 1;
+    while (tiles * splits * 2 <= 256 && k / (splits * 2) >= size_t(4 * bk))
+        splits *= 2;
+    return uint32_t(splits);
+}
+
+// Scratch bytes needed by gemm() for split partials (0 when it writes C directly).
+inline size_t gemm_scratch_bytes(int mode, size_t m, size_t n, size_t k)
+{
+    int const bk = mode == FP_CONTRACT_BF16X3 ? Traits<kBF16x3>::BK : 16;
+    uint32_t const splits = choose_splits(m, n, k, bk);
+    if (splits == 1)
+        return 0;
+    return size_t(splits) * m * n * (mode == FP_CONTRACT_FP64 ? sizeof(double) : sizeof(float));
+}
+
+template <int MODE>
+fp_status launch_gemm(size_t m, size_t n, size_t k, void const *a, void const *b, void *c, void *scratch,
                       hipStream_t stream)
 {
-    dim3 const grid(uint32_t((n + kTile - 1) / kTile), uint32_t((m + kTile - 1) / kTile));
-    switch (mode)
+    typedef typename Traits<MODE>::T T;
+    constexpr int BK = Traits<MODE>::BK;
+    uint32_t const splits = scratch ? choose_splits(m, n, k, BK) : 1;
+    uint64_t k_per = (k + splits - 1) / splits;
+    k_per = (k_per + BK - 1) / BK * BK;
+    dim3 const grid(uint32_t((n + kBN - 1) / kBN), uint32_t((m + kBM - 1) / kBM), splits);
+    T *dst = splits > 1 ? static_cast<T *>(scratch) : static_cast<T *>(c);
+    hipLaunchKernelGGL((k_gemm<MODE>), grid, dim3(256), 0, stream, static_cast<T const *>(a),
+                       static_cast<T const *>(b), dst, uint64_t(m), uint64_t(n), uint64_t(k), k_per);
+    if (splits > 1)
     {
-    case FP_CONTRACT_FP64:
-        hipLaunchKernelGGL((k_gemm_mfma<double, 16>), grid, dim3(256), 0, stream, static_cast<double const *>(a),
-                           static_cast<double const *>(b), static_cast<double *>(c), uint64_t(m), uint64_t(n),
-                           uint64_t(k));
-        break;
-    case FP_CONTRACT_FP32:
-        hipLaunchKernelGGL((k_gemm_mfma<float, 32>), grid, dim3(256), 0, stream, static_cast<float const *>(a),
-                           static_cast<float const *>(b), static_cast<float *>(c), uint64_t(m), uint64_t(n),
-                           uint64_t(k));
-        break;
-    case FP_CONTRACT_BF16X3:
-        hipLaunchKernelGGL((k_gemm_bf16x3<32>), grid, dim3(256), 0, stream, static_cast<float const *>(a),
-                           static_cast<float const *>(b), static_cast<float *>(c), uint64_t(m), uint64_t(n),
-                           uint64_t(k));
-        break;
-    default:
-        g_err = "unknown contraction mode";
-        return FP_ERR_INVALID_ARGUMENT;
+        uint64_t const mn = uint64_t(m) * n;
+        hipLaunchKernelGGL((k_sum_splits<T>), dim3(uint32_t((mn + 255) / 256)), dim3(256), 0, stream,
+                           static_cast<T const *>(scratch), static_cast<T *>(c), mn, splits);
     }
     hipError_t const e = hipGetLastError();
     if (e != hipSuccess)
@@ -205,6 +312,25 @@ inline fp_status gemm(int mode, size_t m, size_t n, size_t k, void const *a, voi
         return FP_ERR_RUNTIME;
     }
     return FP_OK;
+}
+
+// C(m x n) = A(m x k) * B(k x n), row-major.  `scratch` (gemm_scratch_bytes) enables
+// split-K; NULL computes without splitting.
+inline fp_status gemm(int mode, size_t m, size_t n, size_t k, void const *a, void const *b, void *c, void *scratch,
+                      hipStream_t stream)
+{
+    switch (mode)
+    {
+    case FP_CONTRACT_FP64:
+        return launch_gemm<kF64>(m, n, k, a, b, c, scratch, stream);
+    case FP_CONTRACT_FP32:
+        return launch_gemm<kF32>(m, n, k, a, b, c, scratch, stream);
+    case FP_CONTRACT_BF16X3:
+        return launch_gemm<kBF16x3>(m, n, k, a, b, c, scratch, stream);
+    default:
+        g_err = "unknown contraction mode";
+        return FP_ERR_INVALID_ARGUMENT;
+    }
 }
 
 } // namespace fpm

@@ -7,10 +7,13 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <list>
+#include <mutex>
 #include <vector>
 
 #include "fast_pauli_hip.h"
 #include "fp_ev_lds.hpp"
+#include "fp_ev_tiled.hpp"
 #include "fp_kernels.hpp"
 
 namespace fprt
@@ -38,6 +41,32 @@ struct VecGeom
 };
 
 VecGeom vec_geom(int dtype, size_t dim, size_t n_states, void const *a, void const *b);
+
+// Coset-tile geometry of the tiled expectation kernel: 2^lsb states per block,
+// tiles of 2^d rows whose low k bits are consecutive rows and whose dh = d - k
+// high dimensions come from the string group's span.
+struct TileGeom
+{
+    uint32_t lsb, k, d, dh;
+};
+
+// Device tables of the string groups for one (k, dh), cached per operator set.
+struct TileTables
+{
+    uint32_t k = 0, dh = 0, n_groups = 0;
+    void *d_mem = nullptr;
+    fpt::TileGroup *groups = nullptr;
+    uint64_t *btab = nullptr;
+    fpt::TileChunk *chunks = nullptr;
+    fpt::TileStr *strs = nullptr;
+};
+
+TileGeom tile_geom(int dtype, size_t n_qubits, size_t n_states);
+fp_status tile_tables(fp_hip_ops const *ops, TileGeom const &geo, TileTables const **out);
+void release_tiles(fp_hip_ops *ops);
+fp_status choose_op_expect_tiled(fp_hip_ops const *ops, int dtype, size_t n_states, bool *use_tiled);
+fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim,
+                              size_t n_states, int accumulate, hipStream_t st);
 
 } // namespace fprt
 
@@ -99,4 +128,7 @@ struct fp_hip_ops
     double *d_h128 = nullptr;
     float *d_h64 = nullptr;
     uint32_t *d_ny = nullptr;
+    // string groups of the tiled expectation kernel, built on first use per geometry
+    mutable std::mutex tiles_mu;
+    mutable std::list<fprt::TileTables> tiles;
 };

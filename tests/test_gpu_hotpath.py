@@ -160,6 +160,58 @@ def test_op_expectation(hip, dtype, n, N, S):
     assert H.rel_err(got2 - base, exp) <= 10 * tol
 
 
+def _structured_codes(kind, N, n, seed):
+    if kind == "random":
+        return syn.random_codes(N, n, seed=seed)
+    if kind == "weight2":
+        return syn.random_codes_max_weight(N, n, 2, seed=seed)
+    if kind == "diagonal":  # I/Z only: every x mask is 0
+        c = syn.random_codes(N, n, seed=seed)
+        return np.where(c == 0, 0, 3).astype(np.uint8)
+    if kind == "shared_x":  # 4 distinct x masks, many z masks
+        c = syn.random_codes(N, n, seed=seed)
+        xs = (c == 1) | (c == 2)
+        xs = xs[np.arange(N) % 4]
+        zs = syn.random_codes(N, n, seed=seed + 1) >= 2
+        return np.select([xs & zs, xs & ~zs, ~xs & zs], [2, 1, 3], 0).astype(np.uint8)
+    if kind == "duplicates":
+        c = syn.random_codes(max(1, N // 4), n, seed=seed)
+        return np.concatenate([c] * (N // len(c) + 1))[:N]
+    raise ValueError(kind)
+
+
+@pytest.mark.parametrize("kernel", ["stream", "tiled", "auto"])
+@pytest.mark.parametrize("dtype", [C128, C64])
+@pytest.mark.parametrize("kind,n,N,S", [
+    ("random", 2, 5, 1), ("random", 8, 40, 3), ("random", 11, 64, 16), ("random", 13, 200, 7),
+    ("random", 15, 50, 33), ("random", 16, 300, 2), ("random", 18, 64, 16), ("weight2", 14, 300, 5),
+    ("diagonal", 12, 40, 16), ("shared_x", 12, 64, 4), ("duplicates", 10, 33, 1), ("random", 20, 24, 1),
+])
+def test_op_expectation_kernels(hip, monkeypatch, kernel, dtype, kind, n, N, S):
+    """PauliOp::expectation_value through the streaming kernel, the coset-tiled
+    kernel and the automatic choice, on random and structured string sets, at
+    ragged state counts; same bars as above.  Repeated calls are identical."""
+    H.say(f"op_expectation kernel={kernel} {kind} n={n} N={N} S={S} dtype={dtype}")
+    if kernel != "auto":
+        monkeypatch.setenv("FAST_PAULI_EV_KERNEL", kernel)
+    cdt = H.CDT[dtype]
+    codes = _structured_codes(kind, N, n, seed=5 * n + N)
+    strings = syn.codes_to_strings(codes)
+    coeffs = syn.random_coeffs(N, seed=N + 7)
+    ops = _abi.Ops(codes, coeffs)
+    psi = syn.random_states(1 << n, S, seed=n + 3 * S).astype(cdt)
+    got = H.op_expectation(ops, psi, dtype)
+    exp = O.op_expectation(strings, coeffs.astype(cdt).astype(np.complex128),
+                           psi.astype(np.complex128), np.zeros(S, np.complex128))
+    tol = 1e-12 if dtype == C128 else 1e-5
+    assert H.rel_err(got, exp) <= tol
+    assert H.ev_scaled_err(got, exp, coeffs, psi) <= tol
+    assert np.array_equal(H.op_expectation(ops, psi, dtype), got)
+    base = syn.random_coeffs(S, seed=98).astype(cdt)
+    got2 = H.op_expectation(ops, psi, dtype, out0=base)
+    assert H.rel_err(got2 - base, exp) <= 10 * tol
+
+
 def test_expectation_known_answers_and_invariants(hip):
     H.say("expectation known answers")
     a = np.arange(8).astype(np.complex128).reshape(8, 1)
