@@ -124,13 +124,27 @@ def main() -> int:
         ev_ms = [m[1].elapsed_ms(m[2]) for m in marks]
         return total_ms / args.steps, float(np.mean(apply_ms)), float(np.mean(ev_ms))
 
+    plan128 = _abi.op_expectation_plan(ops_ev, C128, dim, S)
+    plan64 = _abi.op_expectation_plan(ops_ev, C64, dim, S)
+    ev_kernel = {_abi.FP_EV_TILED: "fpt::k_op_expect_tiled", _abi.FP_EV_STREAM: "fpk::k_op_expect"}
+
+    def stream_ev_once(dtype, src, evb):
+        """One launch of the streaming expectation kernel (every string reads the state)."""
+        os.environ["FAST_PAULI_EV_KERNEL"] = "stream"
+        try:
+            call("fp_hip_op_expectation", ops_ev.handle, dtype, src.ptr, evb.ptr, dim, S, 0, stream)
+        finally:
+            os.environ.pop("FAST_PAULI_EV_KERNEL", None)
+
     if args.pmc_pass:
         # one launch of each kernel whose HBM bytes the counters measure
         call("fp_hip_copy", out.ptr, psi.ptr, dim * S * 16, stream)
         call("fp_hip_string_apply_batch", ops_apply.handle, 0, C128, psi.ptr, out.ptr, dim, S, 1.0, 0.0, 0, stream)
         call("fp_hip_op_expectation", ops_ev.handle, C128, psi.ptr, ev.ptr, dim, S, 0, stream)
+        stream_ev_once(C128, psi, ev)
         _abi.synchronize()
-        log(f"pmc pass done: copy 4 GiB, apply_batch, expectation_value x{N} (n={n}, {S} states, complex128)")
+        log(f"pmc pass done: copy 4 GiB, apply_batch, expectation_value x{N} ({ev_kernel[plan128['kernel']]}, "
+            f"then fpk::k_op_expect), n={n}, {S} states, complex128")
         return 0
 
     # achievable copy bandwidth on this device (read + write of one 4 GiB buffer)
@@ -143,9 +157,15 @@ def main() -> int:
     ev128 = ev.to_array(np.complex128, (S,))
     bc = 16
     bytes_apply = 2 * dim * S * bc
-    bytes_ev = N * dim * S * bc
+    bytes_ev = plan128["hbm_bytes"]
     value = (1 + N) * S / (ms_step / 1e3)
-    log(f"complex128: step {ms_step:.3f} ms (apply {ms_apply:.3f} ms, EV {ms_ev:.3f} ms)")
+    log(f"complex128: step {ms_step:.3f} ms (apply {ms_apply:.3f} ms, EV {ms_ev:.3f} ms, "
+        f"{ev_kernel[plan128['kernel']]}, {plan128['n_groups']} groups)")
+    # the streaming kernel on the same inputs, for reference (roofline of the per-string form)
+    ms_ev_stream = _time_ms(lambda: stream_ev_once(C128, psi, ev), 2)
+    ev_stream = ev.to_array(np.complex128, (S,))
+    bytes_ev_stream = N * dim * S * bc
+    log(f"complex128 streaming EV kernel: {ms_ev_stream:.3f} ms")
 
     result = {
         "metric": "strings\u00d7states/s & % HBM peak at n=24; p50/p99 latency at fixed QPS",
@@ -165,17 +185,34 @@ def main() -> int:
                    "n_qubits": n, "n_states": S, "n_strings_ev": N, "n_strings_apply": 1,
                    "parallelism": "single GPU"},
         "roofline": {
-            "kernel": "k_op_expect (PauliOp::expectation_value)",
+            "kernel": f"{ev_kernel[plan128['kernel']]} (PauliOp::expectation_value)",
             "bound": "hbm",
             "achieved": bytes_ev / (ms_ev / 1e3) / 1e9,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": bytes_ev / (ms_ev / 1e3) / 1e9 / HBM_PEAK_GBS,
-            "traffic": measured_traffic("k_op_expect", n, S, N),
+            "traffic": measured_traffic(ev_kernel[plan128["kernel"]], n, S, N),
             "algorithmic_bytes_per_launch": bytes_ev,
+            "algorithmic_bytes_rule": "groups x 2^n x S x 16 B: each string group reads the state once"
+                                      if plan128["kernel"] == _abi.FP_EV_TILED else
+                                      "strings x 2^n x S x 16 B: each string reads the state once",
+            "plan": plan128,
             "avg_launch_ms": ms_ev,
             "frac_of_measured_copy_bw": bytes_ev / (ms_ev / 1e3) / 1e9 / copy_gbs,
             "measured_copy_gbs": copy_gbs,
+        },
+        "roofline_ev_streaming": {
+            "kernel": "fpk::k_op_expect (PauliOp::expectation_value, one state read per string)",
+            "bound": "hbm",
+            "achieved": bytes_ev_stream / (ms_ev_stream / 1e3) / 1e9,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": bytes_ev_stream / (ms_ev_stream / 1e3) / 1e9 / HBM_PEAK_GBS,
+            "traffic": measured_traffic("fpk::k_op_expect", n, S, N),
+            "algorithmic_bytes_per_launch": bytes_ev_stream,
+            "avg_launch_ms": ms_ev_stream,
+            "speedup_of_tiled": ms_ev_stream / ms_ev,
+            "rel_diff_vs_default_kernel": float(np.linalg.norm(ev_stream - ev128) / np.linalg.norm(ev128)),
         },
         "roofline_apply_batch": {
             "kernel": "k_string_apply (PauliString::apply_batch)",
@@ -188,7 +225,7 @@ def main() -> int:
             "avg_launch_ms": ms_apply,
             "strings_x_states_per_s": S / (ms_apply / 1e3),
             "frac_of_measured_copy_bw": bytes_apply / (ms_apply / 1e3) / 1e9 / copy_gbs,
-            "traffic": measured_traffic("k_string_apply", n, S, 1),
+            "traffic": measured_traffic("fpk::k_string_apply", n, S, 1),
         },
     }
 
@@ -216,7 +253,9 @@ def main() -> int:
             "apply_batch_ms": ms_apply64,
             "apply_batch_frac_hbm": bytes_apply / 2 / (ms_apply64 / 1e3) / 1e9 / HBM_PEAK_GBS,
             "ev_ms": ms_ev64,
-            "ev_frac_hbm": bytes_ev / 2 / (ms_ev64 / 1e3) / 1e9 / HBM_PEAK_GBS,
+            "ev_kernel": ev_kernel[plan64["kernel"]],
+            "ev_plan": plan64,
+            "ev_frac_hbm": plan64["hbm_bytes"] / (ms_ev64 / 1e3) / 1e9 / HBM_PEAK_GBS,
             "ev_rel_err_vs_fp64_same_inputs": err_norm,
             "ev_max_abs_err_over_sum_abs_coeffs": err_state,
             "input_rounding_rel_effect": err_input,

@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "fp_ev_tiled.hpp"
 #include "fp_kernels.hpp"
 
 namespace fpl
@@ -54,7 +55,7 @@ struct EvStr
     uint32_t z;
     uint32_t imag; // 1: G = Im w, 0: G = Re w
     uint32_t out;  // output row (original string index)
-    uint32_t pad;
+    uint32_t zc;   // coset kernel: z compressed onto the coset-index bits (0 otherwise)
     double scale;  // eps (kAllRows) or 2 eps (kPairs)
 };
 
@@ -214,6 +215,106 @@ __global__ __launch_bounds__(kThreads) void k_ev_lds(typename fpk::VecOf<T, 1>::
             }
         }
     }
+}
+
+// The same evaluation on one coset of a subspace V of row indices, for columns
+// that do not fit in LDS.  The host groups strings so that every x mask of a
+// group lies in its V (dimension d, low k bits included); block (t, y, z) loads
+// coset q = z % n_cos of group g = z / n_cos of column t,
+//     local row a  ->  global row  c_q ^ B(a >> k) ^ (a & lowmask),
+// and runs the group's chunks in local coordinates (local x, local z).  The
+// coset's sign (-1)^{parity(c_q & z)} equals (-1)^{parity(q & zc)}, zc being z
+// compressed onto the coset-index bits (EvStr::zc).  Each (coset, string,
+// state) sum is written to `partial`; k_sum_cosets adds the cosets in order.
+template <class T>
+__global__ __launch_bounds__(kThreads) void k_ev_coset(typename fpk::VecOf<T, 1>::type const *__restrict__ states_T,
+                                                       uint64_t dim, uint32_t d, uint32_t k, uint32_t n_cos,
+                                                       fpt::TileGroup const *__restrict__ groups,
+                                                       uint64_t const *__restrict__ btab,
+                                                       EvChunk const *__restrict__ chunks,
+                                                       EvStr const *__restrict__ strs, uint64_t n_states,
+                                                       uint64_t n_strings, double *__restrict__ partial)
+{
+    typedef typename fpk::VecOf<T, 1>::type V;
+    typedef typename fpk::RowAcc<T>::type A;
+    extern __shared__ __align__(16) unsigned char smem[];
+    V *col = reinterpret_cast<V *>(smem);
+    __shared__ uint32_t next_chunk;
+    uint64_t const t = blockIdx.x;
+    uint32_t const g = blockIdx.z / n_cos, q = blockIdx.z % n_cos;
+    fpt::TileGroup const grp = groups[g];
+    uint64_t const cb = fpt::coset_base(q, k, grp);
+    uint32_t const rows = 1u << d;
+    uint32_t const lowmask = (1u << k) - 1u;
+    V const *src = states_T + t * dim;
+    uint64_t const *bt = btab + grp.btab_off;
+    for (uint32_t i = threadIdx.x; i < rows; i += kThreads)
+        col[i] = src[cb ^ bt[i >> k] ^ uint64_t(i & lowmask)];
+    uint32_t const per = (grp.n_chunks + gridDim.y - 1) / gridDim.y;
+    uint32_t const c_begin = grp.chunk_first + blockIdx.y * per;
+    uint32_t const c_last = grp.chunk_first + grp.n_chunks;
+    uint32_t const c_end = c_begin + per < c_last ? c_begin + per : c_last;
+    if (threadIdx.x == 0)
+        next_chunk = c_begin;
+    __syncthreads();
+    uint32_t const lane = threadIdx.x & 63u;
+    uint32_t const n_blocks64 = rows >> 6;
+    while (true)
+    {
+        uint32_t c = 0;
+        if (lane == 0)
+            c = atomicAdd(&next_chunk, 1u);
+        c = __builtin_amdgcn_readfirstlane(__shfl(c, 0, 64));
+        if (c >= c_end)
+            break;
+        EvChunk const ch = chunks[c];
+        uint32_t zhi[kChunk], lsign[kChunk];
+        bool imag[kChunk];
+        A acc[kChunk];
+#pragma unroll
+        for (int s = 0; s < kChunk; ++s)
+        {
+            acc[s] = A(0);
+            zhi[s] = 0;
+            lsign[s] = 0;
+            imag[s] = false;
+            if (s < int(ch.count))
+            {
+                EvStr const st = strs[ch.first + s];
+                zhi[s] = st.z >> 6;
+                lsign[s] = uint32_t(__popc(lane & st.z & 63u) & 1) << 31;
+                imag[s] = st.imag != 0;
+            }
+        }
+        chunk_rows<T, 4>(col, lane, ch, n_blocks64, zhi, lsign, imag, acc);
+#pragma unroll
+        for (int s = 0; s < kChunk; ++s)
+        {
+            if (s < int(ch.count))
+            {
+                A const sum = wave_sum(acc[s]);
+                if (lane == 0)
+                {
+                    EvStr const st = strs[ch.first + s];
+                    double const v = double(sum) * st.scale;
+                    partial[(uint64_t(q) * n_strings + st.out) * n_states + t] = (__popc(q & st.zc) & 1) ? -v : v;
+                }
+            }
+        }
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_sum_cosets(double const *__restrict__ partial, uint32_t n_cos, uint64_t total,
+                                                    T *__restrict__ ev)
+{
+    uint64_t const i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= total)
+        return;
+    double s = partial[i];
+    for (uint32_t q = 1; q < n_cos; ++q)
+        s += partial[uint64_t(q) * total + i];
+    ev[i] = T(s);
 }
 
 } // namespace fpl

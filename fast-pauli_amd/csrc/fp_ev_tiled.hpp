@@ -37,14 +37,20 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "fp_kernels.hpp"
 
 namespace fpt
 {
 
-constexpr int kThreads = 512;
-constexpr uint32_t kTileBytes = 64u * 1024u;
-constexpr int kMaxDh = 9; // B(a_hi) table: at most 2^9 entries (4 KiB of LDS)
+// Tiles of TKB KiB (64: two 512-thread blocks per CU; 128: one 1024-thread block).
+template <int TKB> struct TileCfg
+{
+    static constexpr uint32_t kBytes = uint32_t(TKB) * 1024u;
+    static constexpr int kThreads = TKB * 8;
+};
+constexpr int kMaxDh = 9; // string-span dimensions per tile (B table of 2^9 entries)
 
 struct TileGroup
 {
@@ -62,15 +68,17 @@ struct TileChunk
     uint32_t h;     // top bit of A
     uint32_t count; // strings in this chunk (1..kChunk), all with local mask A
     uint32_t first; // first TileStr
+    uint32_t need;  // bit 0: some string uses Re w, bit 1: some string uses Im w
+    uint32_t pad[3];
 };
 
 struct TileStr
 {
-    uint64_t z;    // global z mask (coset sign)
-    uint32_t zl;   // local z mask
-    uint32_t sidx; // string index (scale table row)
+    uint64_t zc;   // z compressed onto the coset-index bits: coset sign = parity(q & zc)
+    uint32_t zt;   // local z with bit h removed: row sign of leader p = parity(p & zt)
     uint32_t imag; // 1: Im w (odd nY), 0: Re w
-    uint32_t pad[3];
+    double sre;    // scale c_s * kappa_s
+    double sim;
 };
 
 constexpr int kChunk = 4;
@@ -89,32 +97,39 @@ __device__ __forceinline__ uint64_t coset_base(uint64_t q, uint32_t k, TileGroup
     return v;
 }
 
-template <class T>
-__global__ __launch_bounds__(kThreads, 2) void k_op_expect_tiled(typename fpk::VecOf<T, 1>::type const *__restrict__ states,
-                                                              uint64_t n_states, uint32_t lsb, uint32_t k, uint32_t d,
-                                                              uint32_t ranges, uint64_t per_range, uint64_t n_cosets,
+// Leader rows of a chunk: p in [0, n_rows), a = insert_zero_bit(p, h) (or p
+// itself when A = 0).  A wave evaluates a chunk (or a part of its rows): lane
+// (p_first, t) visits p = p_first | (it << r), with 2^r = 64 / S_blk leader rows
+// per pass.  Since insert_zero_bit is linear over GF(2), a = a_first ^ a_it
+// with a_it wave-uniform, and the row sign parity(p & zt) = parity(p_first & zt)
+// ^ parity(it & (zt >> r)): a per-lane bit applied once per chunk and a
+// wave-uniform bit per pass.
+template <class T, bool WIDE, int TKB>
+__global__ __launch_bounds__(TKB * 8) void k_op_expect_tiled(typename fpk::VecOf<T, 1>::type const *__restrict__ states,
+                                                              uint64_t n_states, uint32_t sshift, uint32_t lsb,
+                                                              uint32_t k, uint32_t d, uint32_t ranges,
+                                                              uint64_t n_cosets,
                                                               TileGroup const *__restrict__ groups,
                                                               uint64_t const *__restrict__ btab,
                                                               TileChunk const *__restrict__ chunks,
                                                               TileStr const *__restrict__ strs,
-                                                              double const *__restrict__ scale,
                                                               fpk::Dbl2 *__restrict__ partial)
 {
     typedef typename fpk::VecOf<T, 1>::type V;
     typedef typename fpk::RowAcc<T>::type Acc;
+    constexpr int kThreads = TileCfg<TKB>::kThreads;
+    constexpr uint32_t kTileBytes = TileCfg<TKB>::kBytes;
     constexpr uint32_t kElems = kTileBytes / sizeof(V);
     constexpr uint32_t EPT = kElems / kThreads;
+    constexpr uint32_t kEsz = sizeof(V);
     constexpr int U = 4;
     extern __shared__ __align__(16) unsigned char smem[];
     V *tile = reinterpret_cast<V *>(smem);
-    uint64_t *bt = reinterpret_cast<uint64_t *>(smem + kTileBytes);
 
     uint32_t const tid = threadIdx.x;
     uint32_t const g = blockIdx.x / ranges, rg = blockIdx.x % ranges;
     TileGroup const grp = groups[g];
-    uint32_t const n_bt = 1u << grp.n_piv;
-    for (uint32_t i = tid; i < n_bt; i += kThreads)
-        bt[i] = btab[grp.btab_off + i];
+    uint64_t const *bt = btab + grp.btab_off;
 
     uint32_t const sblk = 1u << lsb;
     uint32_t const t = tid & (sblk - 1u);
@@ -122,33 +137,64 @@ __global__ __launch_bounds__(kThreads, 2) void k_op_expect_tiled(typename fpk::V
     bool const t_live = s0 + t < n_states;
     uint32_t const tile_elems = (1u << d) << lsb;
     uint32_t const lowmask = (1u << k) - 1u;
-    uint64_t const q0 = uint64_t(rg) * per_range;
-    uint64_t const q1 = q0 + per_range < n_cosets ? q0 + per_range : n_cosets;
-    __syncthreads(); // bt ready
+    // cosets rg, rg + ranges, rg + 2 ranges, ...: at any moment the blocks of a
+    // group work on consecutive cosets, whose rows are neighbours in HBM (the
+    // lowest coset-index bits are the lowest non-pivot row bits)
+    uint64_t const q0 = rg;
+    uint64_t const q_step = ranges;
+
+    // this thread's tile elements: element tid + m * kThreads is local row
+    // a_m = p_first + m * 2^r, global row coset_base ^ roff_m.  With a
+    // power-of-two state count (sshift = log2 S) the element index is
+    // (coset_base << sshift) ^ eoff[m]; otherwise eoff[m] = roff_m.
+    // WIDE = false when 2^n * S < 2^32: offsets fit 32 bits (half the registers).
+    typedef typename std::conditional<WIDE, uint64_t, uint32_t>::type Idx;
+    bool const s_pow2 = sshift < 64u;
+    Idx eoff[EPT];
+#pragma unroll
+    for (uint32_t m = 0; m < EPT; ++m)
+    {
+        uint32_t const e = tid + m * kThreads;
+        uint32_t const a = e >> lsb;
+        bool const live = e < tile_elems && t_live;
+        uint64_t const roff = live ? (bt[a >> k] ^ uint64_t(a & lowmask)) : 0;
+        eoff[m] = Idx(s_pow2 ? ((roff << sshift) | (s0 + t)) : roff);
+    }
 
     V reg[EPT];
     auto fetch = [&](uint64_t q) {
         uint64_t const cb = coset_base(q, k, grp);
+        Idx const cbs = Idx(s_pow2 ? (cb << sshift) : cb);
 #pragma unroll
         for (uint32_t m = 0; m < EPT; ++m)
         {
-            uint32_t const e = tid + m * kThreads;
-            uint32_t const a = e >> lsb;
             reg[m] = V(0);
-            if (e < tile_elems && t_live)
+            if (tid + m * kThreads < tile_elems && t_live)
             {
-                uint64_t const row = cb ^ bt[a >> k] ^ uint64_t(a & lowmask);
-                reg[m] = __builtin_nontemporal_load(&states[row * n_states + s0 + t]);
+                Idx const idx = s_pow2 ? Idx(cbs ^ eoff[m]) : Idx((cbs ^ eoff[m]) * Idx(n_states) + Idx(s0 + t));
+                reg[m] = __builtin_nontemporal_load(&states[idx]);
             }
         }
     };
 
+    // Waves own work units: unit u = (chunk u / split, row part u % split),
+    // u = wave, wave + W, ...; split (a power of two) gives every wave work when a
+    // group has fewer chunks than waves.  Within a wave, lane = (p_first, t):
+    // 2^r = 64 / S_blk leader rows per pass.
+    constexpr uint32_t W = kThreads / 64;
+    uint32_t const wave = tid >> 6, lane = tid & 63u;
+    uint32_t const r = 6u - lsb;
+    uint32_t const p_first = lane >> lsb;
+    uint32_t split = 1;
+    while (grp.n_chunks * split < W && split < 64)
+        split *= 2;
+    uint32_t const n_units = grp.n_chunks * split;
+
     double acc_re = 0.0, acc_im = 0.0;
-    if (q0 < q1)
+    if (q0 < n_cosets)
         fetch(q0);
-    uint32_t const rows_step = kThreads >> lsb;
-    uint32_t const p_first = tid >> lsb;
-    for (uint64_t q = q0; q < q1; ++q)
+    unsigned char const *lds = smem;
+    for (uint64_t q = q0; q < n_cosets; q += q_step)
     {
         __syncthreads(); // the previous tile is no longer read
 #pragma unroll
@@ -159,57 +205,107 @@ __global__ __launch_bounds__(kThreads, 2) void k_op_expect_tiled(typename fpk::V
                 tile[e] = reg[m];
         }
         __syncthreads();
-        if (q + 1 < q1)
-            fetch(q + 1);
-        uint64_t const cb = coset_base(q, k, grp);
-        for (uint32_t c = 0; c < grp.n_chunks; ++c)
+        if (q + q_step < n_cosets)
+            fetch(q + q_step);
+        for (uint32_t u = wave; u < n_units; u += W)
         {
-            TileChunk const ch = chunks[grp.chunk_first + c];
-            uint32_t const n_rows = ch.A ? (1u << (d - 1)) : (1u << d);
-            uint32_t zl[kChunk];
+            TileChunk const ch = chunks[grp.chunk_first + u / split];
+            uint32_t const A = ch.A;
+            uint32_t const h = ch.h;
+            uint32_t const n_rows = A ? (1u << (d - 1)) : (1u << d);
+            uint32_t const n_it = (n_rows >> r) ? (n_rows >> r) : 1u;
+            uint32_t const it_per = (n_it + split - 1) / split;
+            uint32_t const it_begin = (u % split) * it_per;
+            uint32_t const it_end = it_begin + it_per < n_it ? it_begin + it_per : n_it;
+            bool const lane_live = p_first < n_rows;
+            // insert_zero_bit(v, h) = v + (v & hmask) for A != 0
+            uint32_t const hmask = A ? ~((1u << h) - 1u) : 0u;
+            uint32_t const a_first = p_first + (p_first & hmask);
+            uint32_t const off0 = ((a_first << lsb) | t) * kEsz;
+            uint32_t const pxor = (A << lsb) * kEsz;
+            uint32_t zt[kChunk];
             bool imag[kChunk];
             Acc part[kChunk];
 #pragma unroll
             for (int s = 0; s < kChunk; ++s)
             {
                 part[s] = Acc(0);
-                zl[s] = 0;
+                zt[s] = 0;
                 imag[s] = false;
                 if (s < int(ch.count))
                 {
-                    zl[s] = strs[ch.first + s].zl;
+                    zt[s] = strs[ch.first + s].zt;
                     imag[s] = strs[ch.first + s].imag != 0;
                 }
             }
-            for (uint32_t p0 = p_first; p0 < n_rows; p0 += rows_step * U)
+            if (lane_live && it_begin < it_end)
             {
-                V va[U], vb[U];
-                uint32_t aa[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u)
+                if (ch.count == 1)
                 {
-                    uint32_t const p = p0 + uint32_t(u) * rows_step;
-                    uint32_t const pc = p < n_rows ? p : p_first;
-                    uint32_t const a = ch.A ? insert_zero_bit32(pc, ch.h) : pc;
-                    aa[u] = a;
-                    va[u] = tile[(a << lsb) + t];
-                    vb[u] = tile[((a ^ ch.A) << lsb) + t];
-                }
-#pragma unroll
-                for (int u = 0; u < U; ++u)
-                {
-                    if (p0 + uint32_t(u) * rows_step < n_rows)
+                    // one string: only the needed part of w, sign by a scalar +-1
+                    bool const im = imag[0];
+                    uint32_t const zr = zt[0] >> r;
+                    Acc p0 = Acc(0);
+                    for (uint32_t i0 = it_begin; i0 < it_end; i0 += U)
                     {
-                        Acc const ar = Acc(va[u][0]), ai = Acc(va[u][1]), br = Acc(vb[u][0]), bi = Acc(vb[u][1]);
-                        Acc const gre = ar * br + ai * bi;
-                        Acc const gim = ar * bi - ai * br;
+                        V va[U], vb[U];
 #pragma unroll
-                        for (int s = 0; s < kChunk; ++s)
+                        for (int uu = 0; uu < U; ++uu)
                         {
-                            if (s < int(ch.count))
+                            uint32_t const it = i0 + uu < it_end ? i0 + uu : i0;
+                            uint32_t const pit = it << r;
+                            uint32_t const addr = off0 ^ (((pit + (pit & hmask)) << lsb) * kEsz);
+                            va[uu] = *reinterpret_cast<V const *>(lds + addr);
+                            vb[uu] = *reinterpret_cast<V const *>(lds + (addr ^ pxor));
+                        }
+#pragma unroll
+                        for (int uu = 0; uu < U; ++uu)
+                        {
+                            if (i0 + uu < it_end)
                             {
-                                Acc const gv = imag[s] ? gim : gre;
-                                part[s] += (__popc(aa[u] & zl[s]) & 1) ? -gv : gv;
+                                uint32_t const it = i0 + uu;
+                                Acc const ar = Acc(va[uu][0]), ai = Acc(va[uu][1]);
+                                Acc const br = Acc(vb[uu][0]), bi = Acc(vb[uu][1]);
+                                Acc const gv = im ? (ar * bi - ai * br) : (ar * br + ai * bi);
+                                p0 = (__popc(it & zr) & 1) ? p0 - gv : p0 + gv;
+                            }
+                        }
+                    }
+                    part[0] = p0;
+                }
+                else
+                {
+                    for (uint32_t i0 = it_begin; i0 < it_end; i0 += U)
+                    {
+                        V va[U], vb[U];
+#pragma unroll
+                        for (int uu = 0; uu < U; ++uu)
+                        {
+                            uint32_t const it = i0 + uu < it_end ? i0 + uu : i0;
+                            uint32_t const pit = it << r;
+                            uint32_t const addr = off0 ^ (((pit + (pit & hmask)) << lsb) * kEsz);
+                            va[uu] = *reinterpret_cast<V const *>(lds + addr);
+                            vb[uu] = *reinterpret_cast<V const *>(lds + (addr ^ pxor));
+                        }
+#pragma unroll
+                        for (int uu = 0; uu < U; ++uu)
+                        {
+                            if (i0 + uu < it_end)
+                            {
+                                uint32_t const it = i0 + uu;
+                                Acc const ar = Acc(va[uu][0]), ai = Acc(va[uu][1]);
+                                Acc const br = Acc(vb[uu][0]), bi = Acc(vb[uu][1]);
+                                Acc const gre = ar * br + ai * bi;
+                                Acc const gim = ar * bi - ai * br;
+#pragma unroll
+                                for (int s = 0; s < kChunk; ++s)
+                                {
+                                    if (s < int(ch.count))
+                                    {
+                                        Acc const gv = imag[s] ? gim : gre;
+                                        part[s] = (__popc(it & (zt[s] >> r)) & 1) ? part[s] - gv : part[s] + gv;
+                                    }
+                                }
                             }
                         }
                     }
@@ -221,14 +317,10 @@ __global__ __launch_bounds__(kThreads, 2) void k_op_expect_tiled(typename fpk::V
                 if (s < int(ch.count))
                 {
                     TileStr const st = strs[ch.first + s];
-                    double sr = scale[2 * st.sidx], si = scale[2 * st.sidx + 1];
-                    if (fpk::parity64(cb & st.z))
-                    {
-                        sr = -sr;
-                        si = -si;
-                    }
-                    acc_re += sr * double(part[s]);
-                    acc_im += si * double(part[s]);
+                    bool const neg = ((__popc(p_first & zt[s]) ^ __popcll(q & st.zc)) & 1) != 0;
+                    double const v = neg ? -double(part[s]) : double(part[s]);
+                    acc_re += st.sre * v;
+                    acc_im += st.sim * v;
                 }
             }
         }

@@ -586,6 +586,8 @@ size_t strings_expect_partial_bytes(fp_hip_ops const *ops, int dtype, size_t dim
 {
     if (lds_eligible(ops, dtype, dim))
         return dim * n_states * (dtype == FP_C128 ? 16 : 8);
+    if (fprt::ev_coset_eligible(ops, dtype, dim, n_states))
+        return fprt::ev_coset_scratch_bytes(ops, dtype, dim, n_states);
     VecGeom const vg = vec_geom(dtype, dim, n_states, states, states);
     EvGeom const e = ev_geom(vg, dim, ops->n_strings, true, n_states);
     return size_t(e.gx) * ops->n_strings * n_states * sizeof(double);
@@ -639,6 +641,8 @@ fp_status run_strings_expect(fp_hip_ops const *ops, int dtype, void const *state
             return launch_ev_lds<double>(ops, states, ev, dim, n_states, partial, st);
         return launch_ev_lds<float>(ops, states, ev, dim, n_states, partial, st);
     }
+    if (fprt::ev_coset_eligible(ops, dtype, dim, n_states))
+        return fprt::run_ev_coset(ops, dtype, states, ev, dim, n_states, partial, st);
     VecGeom const vg = vec_geom(dtype, dim, n_states, states, states);
     EvGeom const e = ev_geom(vg, dim, N, true, n_states);
     if (dtype == FP_C128)
@@ -1059,6 +1063,7 @@ fp_status fp_hip_ops_set_coeffs(fp_hip_ops *ops, double const *coeffs)
     if (!coeffs)
         return fail(FP_ERR_INVALID_ARGUMENT, "coeffs is NULL");
     (void)hipDeviceSynchronize();
+    fprt::release_tiles(ops); // the tiled expectation tables hold the scaled coefficients
     std::memcpy(ops->coeffs.data(), coeffs, 2 * ops->n_strings * sizeof(double));
     ArenaLayout const L = arena_layout(ops->n_strings);
     fill_coeff_tables(ops, L);

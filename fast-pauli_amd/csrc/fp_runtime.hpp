@@ -47,22 +47,35 @@ VecGeom vec_geom(int dtype, size_t dim, size_t n_states, void const *a, void con
 // high dimensions come from the string group's span.
 struct TileGeom
 {
-    uint32_t lsb, k, d, dh;
+    uint32_t lsb, k, d, dh, tile_kb;
 };
 
-// Device tables of the string groups for one (k, dh), cached per operator set.
+enum
+{
+    kTilesOpExpect = 0, // fpt::TileChunk / fpt::TileStr records (PauliOp expectation)
+    kTilesEvMatrix = 1  // fpl::EvChunk / fpl::EvStr records (per-string EV matrix on cosets)
+};
+
+// Device tables of the string groups of one kind and geometry (k, dh), cached
+// per operator set.
 struct TileTables
 {
-    uint32_t k = 0, dh = 0, n_groups = 0;
+    int kind = 0;
+    uint32_t k = 0, dh = 0, n_groups = 0, max_chunks = 0;
     void *d_mem = nullptr;
     fpt::TileGroup *groups = nullptr;
     uint64_t *btab = nullptr;
-    fpt::TileChunk *chunks = nullptr;
-    fpt::TileStr *strs = nullptr;
+    void *chunks = nullptr;
+    void *strs = nullptr;
 };
 
 TileGeom tile_geom(int dtype, size_t n_qubits, size_t n_states);
-fp_status tile_tables(fp_hip_ops const *ops, TileGeom const &geo, TileTables const **out);
+fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, TileTables const **out);
+// Per-string EV matrix over cosets: for columns that do not fit in LDS whole.
+bool ev_coset_eligible(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_states);
+size_t ev_coset_scratch_bytes(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_states);
+fp_status run_ev_coset(fp_hip_ops const *ops, int dtype, void const *states, void *ev, size_t dim, size_t n_states,
+                       void *scratch, hipStream_t st);
 void release_tiles(fp_hip_ops *ops);
 fp_status choose_op_expect_tiled(fp_hip_ops const *ops, int dtype, size_t n_states, bool *use_tiled);
 fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim,

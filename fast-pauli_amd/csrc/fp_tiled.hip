@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "fast_pauli_hip.h"
@@ -35,6 +36,14 @@ uint32_t log2u(uint64_t v)
 uint32_t parity(uint64_t v)
 {
     return uint32_t(__builtin_popcountll(v) & 1);
+}
+
+// kappa of the pair identity: 1 for x = 0, else 2 eps, eps = +1 for nY = 0, 1 (mod 4), -1 otherwise
+double kappa_of(uint64_t x, uint32_t ny)
+{
+    if (x == 0)
+        return 1.0;
+    return ((ny & 3u) <= 1u) ? 2.0 : -2.0;
 }
 
 uint32_t env_u32(char const *name, uint32_t dflt)
@@ -78,17 +87,23 @@ TileGeom tile_geom(int dtype, size_t n_qubits, size_t n_states)
 {
     uint32_t const esz = dtype == FP_C128 ? 16u : 8u;
     uint32_t const row_target = env_u32("FAST_PAULI_TILE_ROW_BYTES", 256u);
-    uint32_t const elems = fpt::kTileBytes / esz;
+    uint32_t const tile_kb = env_u32("FAST_PAULI_TILE_KB", 64u) >= 128u ? 128u : 64u;
+    uint32_t const elems = tile_kb * 1024u / esz;
     uint32_t sblk = 1;
     while (sblk < n_states && sblk * esz < row_target)
         sblk *= 2;
     TileGeom g{};
+    g.tile_kb = tile_kb;
     g.lsb = log2u(sblk);
     g.d = std::min<uint32_t>(log2u(elems / sblk), uint32_t(n_qubits));
     g.k = 0;
     if (sblk >= n_states) // one block covers every state: 2^k consecutive rows are contiguous
-        while ((uint64_t(1) << g.k) * n_states * esz < 128 && g.k < g.d)
+        while ((uint64_t(1) << g.k) * n_states * esz < 256 && g.k < g.d)
             ++g.k;
+    // FAST_PAULI_TILE_K: at least this many low row bits in every tile (longer contiguous segments)
+    uint32_t const kmin = std::min<uint32_t>(env_u32("FAST_PAULI_TILE_K", 0), g.d);
+    if (sblk >= n_states)
+        g.k = std::max(g.k, kmin);
     g.dh = g.d - g.k;
     if (g.dh > uint32_t(fpt::kMaxDh))
     {
@@ -98,28 +113,40 @@ TileGeom tile_geom(int dtype, size_t n_qubits, size_t n_states)
     return g;
 }
 
-// Builds (or returns the cached) tile tables of `ops` for geometry (k, dh).
-fp_status tile_tables(fp_hip_ops const *ops, TileGeom const &geo, TileTables const **out)
+namespace
 {
-    std::lock_guard<std::mutex> lock(ops->tiles_mu);
-    for (auto const &t : ops->tiles)
-        if (t.k == geo.k && t.dh == geo.dh)
-        {
-            *out = &t;
-            return FP_OK;
-        }
+
+struct LocalStr
+{
+    uint32_t s;  // string index
+    uint32_t A;  // local x mask
+    uint32_t Z;  // local z mask
+    uint64_t zc; // z compressed onto the coset-index bits
+};
+
+struct HostGroup
+{
+    std::vector<uint64_t> b;   // basis, ascending pivot order
+    std::vector<uint32_t> piv; // pivots, ascending
+    std::vector<LocalStr> mem; // members, sorted by local x mask
+};
+
+// Greedy grouping of the strings of `ops` into subspaces of dimension k + dh
+// (the low k bits plus dh reduced basis vectors), in x order.
+fp_status group_strings(fp_hip_ops const *ops, uint32_t k, uint32_t dh, std::vector<HostGroup> &out)
+{
     size_t const N = ops->n_strings, n = ops->n_qubits;
-    uint32_t const k = geo.k, dh = geo.dh;
     uint64_t const lowmask = (uint64_t(1) << k) - 1;
+    if (k + dh > n)
+        return fail(FP_ERR_RUNTIME, "tile geometry k=%u dh=%u exceeds %zu qubits", k, dh, n);
     std::vector<uint32_t> order(N);
     for (size_t i = 0; i < N; ++i)
         order[i] = uint32_t(i);
     std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return ops->x[a] < ops->x[b]; });
-
-    std::vector<fpt::TileGroup> groups;
-    std::vector<uint64_t> btab;
-    std::vector<fpt::TileChunk> chunks;
-    std::vector<fpt::TileStr> strs;
+    if (dh == 0)
+        for (uint32_t s : order)
+            if (ops->x[s] & ~lowmask)
+                return fail(FP_ERR_RUNTIME, "tile geometry has no room for x mask bits above bit %u", k);
     Span span;
     std::vector<uint32_t> members;
     auto close = [&]() {
@@ -131,71 +158,43 @@ fp_status tile_tables(fp_hip_ops const *ops, TileGeom const &geo, TileTables con
             if (r)
                 span.add(r);
         }
-        // basis in ascending pivot order: bit i of a_hi <-> i-th smallest pivot
         std::vector<size_t> idx(span.b.size());
         for (size_t i = 0; i < idx.size(); ++i)
             idx[i] = i;
         std::sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return span.piv[a] < span.piv[b]; });
-        fpt::TileGroup g{};
-        g.n_piv = uint32_t(idx.size());
-        g.btab_off = uint32_t(btab.size());
-        for (size_t i = 0; i < idx.size(); ++i)
-            (i < 8 ? g.piv_lo : g.piv_hi) |= uint64_t(span.piv[idx[i]]) << (8 * (i & 7));
-        size_t const n_bt = size_t(1) << g.n_piv;
-        size_t const base = btab.size();
-        btab.resize(base + n_bt, 0);
-        for (size_t a = 1; a < n_bt; ++a)
-            btab[base + a] = btab[base + (a & (a - 1))] ^ span.b[idx[__builtin_ctzll(a)]];
-        struct Loc
+        HostGroup g;
+        uint64_t pivmask = 0;
+        for (size_t i : idx)
         {
-            uint32_t A, Z, s;
-        };
-        std::vector<Loc> loc;
+            g.b.push_back(span.b[i]);
+            g.piv.push_back(span.piv[i]);
+            pivmask |= uint64_t(1) << span.piv[i];
+        }
+        // coset-index bits: positions >= k that are not pivots, ascending
+        std::vector<uint32_t> cpos;
+        for (uint32_t i = k; i < n; ++i)
+            if (!((pivmask >> i) & 1u))
+                cpos.push_back(i);
         for (uint32_t s : members)
         {
-            uint64_t const xh = ops->x[s] & ~lowmask;
+            uint64_t const xh = ops->x[s] & ~lowmask, z = ops->z[s];
             uint32_t alpha = 0, zeta = 0;
-            for (size_t i = 0; i < idx.size(); ++i)
+            uint64_t zc = 0;
+            for (size_t i = 0; i < g.b.size(); ++i)
             {
-                alpha |= uint32_t((xh >> span.piv[idx[i]]) & 1u) << i;
-                zeta |= parity(span.b[idx[i]] & ops->z[s]) << i;
+                alpha |= uint32_t((xh >> g.piv[i]) & 1u) << i;
+                zeta |= parity(g.b[i] & z) << i;
             }
-            loc.push_back(Loc{(alpha << k) | uint32_t(ops->x[s] & lowmask), (zeta << k) | uint32_t(ops->z[s] & lowmask),
-                              s});
+            for (size_t i = 0; i < cpos.size(); ++i)
+                zc |= ((z >> cpos[i]) & 1u) << i;
+            g.mem.push_back(LocalStr{s, (alpha << k) | uint32_t(ops->x[s] & lowmask),
+                                     (zeta << k) | uint32_t(z & lowmask), zc});
         }
-        std::stable_sort(loc.begin(), loc.end(), [](Loc const &a, Loc const &b) { return a.A < b.A; });
-        g.chunk_first = uint32_t(chunks.size());
-        for (size_t i = 0; i < loc.size();)
-        {
-            size_t j = i;
-            while (j < loc.size() && j - i < size_t(fpt::kChunk) && loc[j].A == loc[i].A)
-                ++j;
-            fpt::TileChunk c{};
-            c.A = loc[i].A;
-            c.h = c.A ? 31u - uint32_t(__builtin_clz(c.A)) : 0u;
-            c.count = uint32_t(j - i);
-            c.first = uint32_t(strs.size());
-            for (size_t m = i; m < j; ++m)
-            {
-                fpt::TileStr st{};
-                st.z = ops->z[loc[m].s];
-                st.zl = loc[m].Z;
-                st.sidx = loc[m].s;
-                st.imag = ops->ny[loc[m].s] & 1u;
-                strs.push_back(st);
-            }
-            chunks.push_back(c);
-            i = j;
-        }
-        g.n_chunks = uint32_t(chunks.size()) - g.chunk_first;
-        groups.push_back(g);
+        std::stable_sort(g.mem.begin(), g.mem.end(), [](LocalStr const &a, LocalStr const &b) { return a.A < b.A; });
+        out.push_back(std::move(g));
         span = Span{};
         members.clear();
     };
-    if (dh == 0)
-        for (uint32_t s : order)
-            if (ops->x[s] & ~lowmask)
-                return fail(FP_ERR_RUNTIME, "tile geometry has no room for x mask bits above bit %u", k);
     for (uint32_t s : order)
     {
         uint64_t const r = span.reduce(ops->x[s] & ~lowmask);
@@ -211,22 +210,155 @@ fp_status tile_tables(fp_hip_ops const *ops, TileGeom const &geo, TileTables con
         members.push_back(s);
     }
     close();
+    for (auto const &g : out)
+        if (g.b.size() != dh)
+            return fail(FP_ERR_RUNTIME, "string group has dimension %zu, expected %u", g.b.size(), dh);
+    return FP_OK;
+}
+
+fpt::TileGroup group_header(HostGroup const &hg, std::vector<uint64_t> &btab)
+{
+    fpt::TileGroup g{};
+    g.n_piv = uint32_t(hg.b.size());
+    g.btab_off = uint32_t(btab.size());
+    for (size_t i = 0; i < hg.piv.size(); ++i)
+        (i < 8 ? g.piv_lo : g.piv_hi) |= uint64_t(hg.piv[i]) << (8 * (i & 7));
+    size_t const n_bt = size_t(1) << g.n_piv;
+    size_t const base = btab.size();
+    btab.resize(base + n_bt, 0);
+    for (size_t a = 1; a < n_bt; ++a)
+        btab[base + a] = btab[base + (a & (a - 1))] ^ hg.b[__builtin_ctzll(a)];
+    return g;
+}
+
+} // namespace
+
+// Builds (or returns the cached) tables of `ops` of one kind for geometry (k, dh).
+fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, TileTables const **out)
+{
+    std::lock_guard<std::mutex> lock(ops->tiles_mu);
+    for (auto const &t : ops->tiles)
+        if (t.kind == kind && t.k == k && t.dh == dh)
+        {
+            *out = &t;
+            return FP_OK;
+        }
+    std::vector<HostGroup> hgs;
+    fp_status const st = group_strings(ops, k, dh, hgs);
+    if (st != FP_OK)
+        return st;
+    std::vector<fpt::TileGroup> groups;
+    std::vector<uint64_t> btab;
+    std::vector<unsigned char> chunks, strs; // raw bytes of the kind's chunk / string records
+    size_t n_chunks = 0;
+    for (auto const &hg : hgs)
+    {
+        fpt::TileGroup g = group_header(hg, btab);
+        g.chunk_first = uint32_t(n_chunks);
+        if (kind == kTilesOpExpect)
+        {
+            for (size_t i = 0; i < hg.mem.size();)
+            {
+                size_t j = i;
+                while (j < hg.mem.size() && j - i < size_t(fpt::kChunk) && hg.mem[j].A == hg.mem[i].A)
+                    ++j;
+                fpt::TileChunk c{};
+                c.A = hg.mem[i].A;
+                c.h = c.A ? 31u - uint32_t(__builtin_clz(c.A)) : 0u;
+                c.count = uint32_t(j - i);
+                c.first = uint32_t(strs.size() / sizeof(fpt::TileStr));
+                for (size_t m = i; m < j; ++m)
+                {
+                    uint32_t const sidx = hg.mem[m].s;
+                    uint32_t const Z = hg.mem[m].Z;
+                    fpt::TileStr r{};
+                    r.zc = hg.mem[m].zc;
+                    // leader p maps to a = insert_zero_bit(p, h): drop bit h of Z
+                    r.zt = c.A ? ((Z & ((1u << c.h) - 1u)) | ((Z >> (c.h + 1)) << c.h)) : Z;
+                    r.imag = ops->ny[sidx] & 1u;
+                    c.need |= r.imag ? 2u : 1u;
+                    double const kap = kappa_of(ops->x[sidx], ops->ny[sidx]);
+                    r.sre = ops->coeffs[2 * sidx] * kap;
+                    r.sim = ops->coeffs[2 * sidx + 1] * kap;
+                    strs.insert(strs.end(), reinterpret_cast<unsigned char const *>(&r),
+                                reinterpret_cast<unsigned char const *>(&r) + sizeof(r));
+                }
+                chunks.insert(chunks.end(), reinterpret_cast<unsigned char const *>(&c),
+                              reinterpret_cast<unsigned char const *>(&c) + sizeof(c));
+                ++n_chunks;
+                i = j;
+            }
+        }
+        else
+        {
+            // per-string EV matrix: chunks of <= fpl::kChunk strings with equal local x,
+            // costliest first (the LDS kernel's chunk records)
+            struct Pending
+            {
+                fpl::EvChunk c;
+                std::vector<fpl::EvStr> s;
+                double cost;
+            };
+            std::vector<Pending> pend;
+            for (size_t i = 0; i < hg.mem.size();)
+            {
+                size_t j = i;
+                while (j < hg.mem.size() && j - i < size_t(fpl::kChunk) && hg.mem[j].A == hg.mem[i].A)
+                    ++j;
+                Pending p{};
+                p.c.x = hg.mem[i].A;
+                p.c.h = p.c.x ? 31u - uint32_t(__builtin_clz(p.c.x)) : 0u;
+                p.c.mode = (p.c.x != 0 && p.c.h >= 6) ? fpl::kPairs : fpl::kAllRows;
+                p.c.count = uint32_t(j - i);
+                for (size_t m = i; m < j; ++m)
+                {
+                    uint32_t const sidx = hg.mem[m].s;
+                    uint32_t const ny = ops->ny[sidx];
+                    double const eps = ((ny & 3u) <= 1u) ? 1.0 : -1.0;
+                    p.s.push_back(fpl::EvStr{hg.mem[m].Z, ny & 1u, sidx, uint32_t(hg.mem[m].zc),
+                                             eps * (p.c.mode == fpl::kPairs ? 2.0 : 1.0)});
+                }
+                p.cost = double(p.c.count + 2) * (p.c.mode == fpl::kPairs ? 1.0 : 2.0);
+                pend.push_back(std::move(p));
+                i = j;
+            }
+            std::stable_sort(pend.begin(), pend.end(),
+                             [](Pending const &a, Pending const &b) { return a.cost > b.cost; });
+            for (auto &p : pend)
+            {
+                p.c.first = uint32_t(strs.size() / sizeof(fpl::EvStr));
+                for (auto const &e : p.s)
+                    strs.insert(strs.end(), reinterpret_cast<unsigned char const *>(&e),
+                                reinterpret_cast<unsigned char const *>(&e) + sizeof(e));
+                chunks.insert(chunks.end(), reinterpret_cast<unsigned char const *>(&p.c),
+                              reinterpret_cast<unsigned char const *>(&p.c) + sizeof(p.c));
+                ++n_chunks;
+            }
+        }
+        g.n_chunks = uint32_t(n_chunks) - g.chunk_first;
+        groups.push_back(g);
+    }
+    uint32_t max_chunks = 0;
+    for (auto const &g : groups)
+        max_chunks = std::max(max_chunks, g.n_chunks);
 
     TileTables t{};
+    t.kind = kind;
     t.k = k;
     t.dh = dh;
     t.n_groups = uint32_t(groups.size());
+    t.max_chunks = max_chunks;
     auto up = [](size_t v) { return (v + 255) / 256 * 256; };
     size_t const o_groups = 0;
     size_t const o_btab = o_groups + up(groups.size() * sizeof(fpt::TileGroup));
     size_t const o_chunks = o_btab + up(btab.size() * sizeof(uint64_t));
-    size_t const o_strs = o_chunks + up(chunks.size() * sizeof(fpt::TileChunk));
-    size_t const total = o_strs + up(strs.size() * sizeof(fpt::TileStr));
+    size_t const o_strs = o_chunks + up(chunks.size());
+    size_t const total = o_strs + up(strs.size());
     std::vector<unsigned char> h(total, 0);
     std::memcpy(h.data() + o_groups, groups.data(), groups.size() * sizeof(fpt::TileGroup));
     std::memcpy(h.data() + o_btab, btab.data(), btab.size() * sizeof(uint64_t));
-    std::memcpy(h.data() + o_chunks, chunks.data(), chunks.size() * sizeof(fpt::TileChunk));
-    std::memcpy(h.data() + o_strs, strs.data(), strs.size() * sizeof(fpt::TileStr));
+    std::memcpy(h.data() + o_chunks, chunks.data(), chunks.size());
+    std::memcpy(h.data() + o_strs, strs.data(), strs.size());
     FP_HIP(hipMalloc(&t.d_mem, total));
     hipError_t const e = hipMemcpy(t.d_mem, h.data(), total, hipMemcpyHostToDevice);
     if (e != hipSuccess)
@@ -237,8 +369,8 @@ fp_status tile_tables(fp_hip_ops const *ops, TileGeom const &geo, TileTables con
     auto *base = static_cast<unsigned char *>(t.d_mem);
     t.groups = reinterpret_cast<fpt::TileGroup *>(base + o_groups);
     t.btab = reinterpret_cast<uint64_t *>(base + o_btab);
-    t.chunks = reinterpret_cast<fpt::TileChunk *>(base + o_chunks);
-    t.strs = reinterpret_cast<fpt::TileStr *>(base + o_strs);
+    t.chunks = base + o_chunks;
+    t.strs = base + o_strs;
     ops->tiles.push_back(t);
     *out = &ops->tiles.back();
     return FP_OK;
@@ -266,7 +398,7 @@ fp_status choose_op_expect_tiled(fp_hip_ops const *ops, int dtype, size_t n_stat
         return FP_OK;
     TileGeom const geo = tile_geom(dtype, ops->n_qubits, n_states);
     TileTables const *tt = nullptr;
-    fp_status const st = tile_tables(ops, geo, &tt);
+    fp_status const st = tile_tables(ops, kTilesOpExpect, geo.k, geo.dh, &tt);
     if (st != FP_OK)
         return st;
     *use_tiled = forced || double(tt->n_groups) <= 0.8 * double(ops->n_strings);
@@ -278,19 +410,20 @@ fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *stat
 {
     TileGeom const geo = tile_geom(dtype, ops->n_qubits, n_states);
     TileTables const *tt = nullptr;
-    fp_status s = tile_tables(ops, geo, &tt);
+    fp_status s = tile_tables(ops, kTilesOpExpect, geo.k, geo.dh, &tt);
     if (s != FP_OK)
         return s;
     uint64_t const n_cosets = uint64_t(dim) >> geo.d;
     uint32_t const chunks_y = uint32_t((n_states + (size_t(1) << geo.lsb) - 1) >> geo.lsb);
     if (chunks_y > 65535u)
         return fail(FP_ERR_INVALID_ARGUMENT, "too many states for the tiled expectation kernel");
-    // about 2048 blocks in total, each over a contiguous range of cosets
-    uint64_t const target = 2048;
+    // up to 8192 blocks (FAST_PAULI_TILE_BLOCKS) with at least ~16 tiles each; the
+    // `ranges` blocks of a group take every ranges-th coset
+    uint64_t const tiles = uint64_t(tt->n_groups) * chunks_y * n_cosets;
+    uint64_t const target = std::max<uint64_t>(256, std::min<uint64_t>(env_u32("FAST_PAULI_TILE_BLOCKS", 8192), tiles / 16));
     uint64_t want = (target + uint64_t(tt->n_groups) * chunks_y - 1) / (uint64_t(tt->n_groups) * chunks_y);
     want = std::max<uint64_t>(1, std::min<uint64_t>(want, n_cosets));
-    uint64_t const per_range = (n_cosets + want - 1) / want;
-    uint32_t const ranges = uint32_t((n_cosets + per_range - 1) / per_range);
+    uint32_t const ranges = uint32_t(std::min<uint64_t>(want, 0x7fffffffu));
     uint64_t const n_parts = uint64_t(tt->n_groups) * ranges;
     if (n_parts > 0x7fffffffu)
         return fail(FP_ERR_INVALID_ARGUMENT, "too many tile groups");
@@ -298,43 +431,204 @@ fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *stat
     if ((s = scratch_get(st, n_parts * n_states * sizeof(fpk::Dbl2), &scratch)) != FP_OK)
         return s;
     auto *partial = static_cast<fpk::Dbl2 *>(scratch);
-    size_t const lds = fpt::kTileBytes + (size_t(1) << geo.dh) * sizeof(uint64_t);
+    size_t const lds = size_t(geo.tile_kb) * 1024u;
+    uint32_t const sshift = (n_states & (n_states - 1)) == 0 ? log2u(n_states) : 64u;
+    // 64-bit element offsets when the state has 2^32 elements or more (FAST_PAULI_TILE_WIDE=1
+    // forces them, so that tests cover both forms at small sizes)
+    bool const wide = (uint64_t(dim) * n_states) >> 32 != 0 || env_u32("FAST_PAULI_TILE_WIDE", 0) == 1;
     static std::once_flag attr_once;
     static hipError_t attr_err = hipSuccess;
     std::call_once(attr_once, [] {
-        hipError_t e1 = hipFuncSetAttribute(reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<double>),
-                                            hipFuncAttributeMaxDynamicSharedMemorySize,
-                                            int(fpt::kTileBytes + (1u << fpt::kMaxDh) * sizeof(uint64_t)));
-        hipError_t e2 = hipFuncSetAttribute(reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<float>),
-                                            hipFuncAttributeMaxDynamicSharedMemorySize,
-                                            int(fpt::kTileBytes + (1u << fpt::kMaxDh) * sizeof(uint64_t)));
-        attr_err = e1 != hipSuccess ? e1 : e2;
+        struct F
+        {
+            void const *fn;
+            int bytes;
+        };
+        F const fns[] = {{reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<double, false, 64>), 64 << 10},
+                         {reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<double, true, 64>), 64 << 10},
+                         {reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<float, false, 64>), 64 << 10},
+                         {reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<float, true, 64>), 64 << 10},
+                         {reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<double, false, 128>), 128 << 10},
+                         {reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<double, true, 128>), 128 << 10},
+                         {reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<float, false, 128>), 128 << 10},
+                         {reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<float, true, 128>), 128 << 10}};
+        for (F const &f : fns)
+        {
+            hipError_t const e = hipFuncSetAttribute(f.fn, hipFuncAttributeMaxDynamicSharedMemorySize, f.bytes);
+            if (e != hipSuccess && attr_err == hipSuccess)
+                attr_err = e;
+        }
     });
     if (attr_err != hipSuccess)
         return fail(FP_ERR_RUNTIME, "hipFuncSetAttribute failed: %s", hipGetErrorString(attr_err));
     dim3 const grid(uint32_t(n_parts), chunks_y);
+    auto launch = [&](auto kfn, int threads, auto const *typed_states, auto *typed_out) -> fp_status {
+        hipLaunchKernelGGL(kfn, grid, dim3(threads), lds, st, typed_states, uint64_t(n_states), sshift, geo.lsb, geo.k,
+                           geo.d, ranges, n_cosets, tt->groups, tt->btab,
+                           static_cast<fpt::TileChunk const *>(tt->chunks), static_cast<fpt::TileStr const *>(tt->strs),
+                           partial);
+        FP_LAUNCHED();
+        typedef std::remove_pointer_t<decltype(typed_out)> OutT;
+        hipLaunchKernelGGL(fpk::k_reduce_partials<OutT>, dim3(uint32_t(n_states)), dim3(fpk::kBlock), 0, st, partial,
+                           n_parts, uint64_t(n_states), typed_out, accumulate, 1.0, 0.0);
+        FP_LAUNCHED();
+        return FP_OK;
+    };
     if (dtype == FP_C128)
     {
         typedef fpk::VecOf<double, 1>::type V;
-        hipLaunchKernelGGL(fpt::k_op_expect_tiled<double>, grid, dim3(fpt::kThreads), lds, st,
-                           static_cast<V const *>(states), uint64_t(n_states), geo.lsb, geo.k, geo.d, ranges,
-                           per_range, n_cosets, tt->groups, tt->btab, tt->chunks, tt->strs, ops->d_scale, partial);
-        FP_LAUNCHED();
-        hipLaunchKernelGGL(fpk::k_reduce_partials<double>, dim3(uint32_t(n_states)), dim3(fpk::kBlock), 0, st,
-                           partial, n_parts, uint64_t(n_states), static_cast<double *>(out), accumulate, 1.0, 0.0);
+        auto const *sp = static_cast<V const *>(states);
+        auto *op = static_cast<double *>(out);
+        if (geo.tile_kb == 128)
+            return wide ? launch(&fpt::k_op_expect_tiled<double, true, 128>, 1024, sp, op)
+                        : launch(&fpt::k_op_expect_tiled<double, false, 128>, 1024, sp, op);
+        return wide ? launch(&fpt::k_op_expect_tiled<double, true, 64>, 512, sp, op)
+                    : launch(&fpt::k_op_expect_tiled<double, false, 64>, 512, sp, op);
     }
-    else
-    {
-        typedef fpk::VecOf<float, 1>::type V;
-        hipLaunchKernelGGL(fpt::k_op_expect_tiled<float>, grid, dim3(fpt::kThreads), lds, st,
-                           static_cast<V const *>(states), uint64_t(n_states), geo.lsb, geo.k, geo.d, ranges,
-                           per_range, n_cosets, tt->groups, tt->btab, tt->chunks, tt->strs, ops->d_scale, partial);
-        FP_LAUNCHED();
-        hipLaunchKernelGGL(fpk::k_reduce_partials<float>, dim3(uint32_t(n_states)), dim3(fpk::kBlock), 0, st,
-                           partial, n_parts, uint64_t(n_states), static_cast<float *>(out), accumulate, 1.0, 0.0);
-    }
+    typedef fpk::VecOf<float, 1>::type V;
+    auto const *sp = static_cast<V const *>(states);
+    auto *op = static_cast<float *>(out);
+    if (geo.tile_kb == 128)
+        return wide ? launch(&fpt::k_op_expect_tiled<float, true, 128>, 1024, sp, op)
+                    : launch(&fpt::k_op_expect_tiled<float, false, 128>, 1024, sp, op);
+    return wide ? launch(&fpt::k_op_expect_tiled<float, true, 64>, 512, sp, op)
+                : launch(&fpt::k_op_expect_tiled<float, false, 64>, 512, sp, op);
+}
+
+// ---------------------------------------------------------------------------
+// Per-string EV matrix on cosets (columns larger than LDS)
+// ---------------------------------------------------------------------------
+namespace
+{
+
+struct CosetGeom
+{
+    uint32_t d, k, n_cos;
+};
+
+constexpr size_t kColumnLds = 160 * 1024 - 256;
+
+CosetGeom coset_geom(int dtype, size_t n_qubits)
+{
+    size_t const esz = dtype == FP_C128 ? 16 : 8;
+    CosetGeom g{};
+    g.d = log2u(kColumnLds / esz);             // 13 (complex128) or 14 (complex64)
+    g.k = dtype == FP_C128 ? 3u : 4u;          // 128-byte runs of consecutive rows
+    g.n_cos = n_qubits > g.d && n_qubits - g.d < 31 ? 1u << (n_qubits - g.d) : 0u;
+    return g;
+}
+
+size_t round256(size_t v)
+{
+    return (v + 255) / 256 * 256;
+}
+
+} // namespace
+
+bool ev_coset_eligible(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_states)
+{
+    char const *v = std::getenv("FAST_PAULI_EV_MATRIX_KERNEL");
+    if (v && std::strcmp(v, "stream") == 0)
+        return false;
+    CosetGeom const g = coset_geom(dtype, ops->n_qubits);
+    if (g.n_cos == 0 || g.n_cos > 16 || (size_t(1) << ops->n_qubits) != dim)
+        return false;
+    return double(g.n_cos) * double(ops->n_strings) * double(n_states) * 8.0 <= double(size_t(1) << 32) &&
+           n_states <= 0x7fffffffu;
+}
+
+size_t ev_coset_scratch_bytes(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_states)
+{
+    CosetGeom const g = coset_geom(dtype, ops->n_qubits);
+    return round256(dim * n_states * (dtype == FP_C128 ? 16 : 8)) +
+           size_t(g.n_cos) * ops->n_strings * n_states * sizeof(double);
+}
+
+template <class T>
+fp_status launch_ev_coset(fp_hip_ops const *ops, TileTables const *tt, CosetGeom const &g, void const *states,
+                          void *ev, size_t dim, size_t n_states, void *scratch, hipStream_t st)
+{
+    typedef typename fpk::VecOf<T, 1>::type V;
+    V *transposed = static_cast<V *>(scratch);
+    auto *partial =
+        reinterpret_cast<double *>(static_cast<unsigned char *>(scratch) + round256(dim * n_states * sizeof(V)));
+    dim3 const tg(uint32_t((n_states + 31) / 32), uint32_t((dim + 31) / 32));
+    hipLaunchKernelGGL(fpl::k_transpose<V>, tg, dim3(256), 0, st, static_cast<V const *>(states), transposed,
+                       uint64_t(dim), uint64_t(n_states));
+    FP_LAUNCHED();
+    static std::once_flag once;
+    static hipError_t attr = hipSuccess;
+    std::call_once(once, [] {
+        attr = hipFuncSetAttribute(reinterpret_cast<void const *>(&fpl::k_ev_coset<T>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(kColumnLds));
+    });
+    if (attr != hipSuccess)
+        return fail(FP_ERR_RUNTIME, "hipFuncSetAttribute failed: %s", hipGetErrorString(attr));
+    uint32_t gy = 1;
+    if (n_states < 1024)
+        gy = uint32_t(std::max<size_t>(1, std::min<size_t>(tt->max_chunks, (1024 + n_states - 1) / n_states)));
+    size_t const lds = (size_t(1) << g.d) * sizeof(V);
+    dim3 const grid(uint32_t(n_states), gy, tt->n_groups * g.n_cos);
+    hipLaunchKernelGGL(fpl::k_ev_coset<T>, grid, dim3(fpl::kThreads), lds, st, transposed, uint64_t(dim), g.d, g.k,
+                       g.n_cos, tt->groups, tt->btab, static_cast<fpl::EvChunk const *>(tt->chunks),
+                       static_cast<fpl::EvStr const *>(tt->strs), uint64_t(n_states), uint64_t(ops->n_strings),
+                       partial);
+    FP_LAUNCHED();
+    uint64_t const total = uint64_t(ops->n_strings) * n_states;
+    hipLaunchKernelGGL(fpl::k_sum_cosets<T>, dim3(uint32_t((total + 255) / 256)), dim3(256), 0, st, partial, g.n_cos,
+                       total, static_cast<T *>(ev));
     FP_LAUNCHED();
     return FP_OK;
 }
 
+fp_status run_ev_coset(fp_hip_ops const *ops, int dtype, void const *states, void *ev, size_t dim, size_t n_states,
+                       void *scratch, hipStream_t st)
+{
+    CosetGeom const g = coset_geom(dtype, ops->n_qubits);
+    TileTables const *tt = nullptr;
+    fp_status const s = tile_tables(ops, kTilesEvMatrix, g.k, g.d - g.k, &tt);
+    if (s != FP_OK)
+        return s;
+    if (tt->n_groups * g.n_cos > 65535u)
+        return fail(FP_ERR_INVALID_ARGUMENT, "too many string groups for the coset kernel");
+    if (dtype == FP_C128)
+        return launch_ev_coset<double>(ops, tt, g, states, ev, dim, n_states, scratch, st);
+    return launch_ev_coset<float>(ops, tt, g, states, ev, dim, n_states, scratch, st);
+}
+
 } // namespace fprt
+
+extern "C" fp_status fp_hip_op_expectation_plan(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_states,
+                                                fp_hip_ev_plan *plan)
+{
+    if (!ops || !plan)
+        return fail(FP_ERR_INVALID_ARGUMENT, "NULL ops or plan");
+    if (dtype != FP_C128 && dtype != FP_C64)
+        return fail(FP_ERR_INVALID_ARGUMENT, "unknown dtype %d", dtype);
+    if (n_states == 0 || dim != (size_t(1) << ops->n_qubits))
+        return fail(FP_ERR_INVALID_ARGUMENT, "shape (%zu, %zu) does not match %zu qubits", dim, n_states,
+                    ops->n_qubits);
+    bool tiled = false;
+    fp_status st = fprt::choose_op_expect_tiled(ops, dtype, n_states, &tiled);
+    if (st != FP_OK)
+        return st;
+    uint64_t const state_bytes = uint64_t(dim) * n_states * (dtype == FP_C128 ? 16u : 8u);
+    *plan = fp_hip_ev_plan{};
+    if (!tiled)
+    {
+        plan->kernel = FP_EV_STREAM;
+        plan->n_groups = uint32_t(ops->n_strings);
+        plan->hbm_bytes = state_bytes * ops->n_strings;
+        return FP_OK;
+    }
+    fprt::TileGeom const geo = fprt::tile_geom(dtype, ops->n_qubits, n_states);
+    fprt::TileTables const *tt = nullptr;
+    if ((st = fprt::tile_tables(ops, fprt::kTilesOpExpect, geo.k, geo.dh, &tt)) != FP_OK)
+        return st;
+    plan->kernel = FP_EV_TILED;
+    plan->n_groups = tt->n_groups;
+    plan->tile_rows_log2 = geo.d;
+    plan->states_per_block = 1u << geo.lsb;
+    plan->hbm_bytes = state_bytes * tt->n_groups;
+    return FP_OK;
+}

@@ -85,12 +85,32 @@ _SIGS = {
     "fp_hip_string_expectation": [_vp, _sz, _i, _vp, _vp, _sz, _sz, _d, _d, _i, _vp],
     "fp_hip_op_apply": [_vp, _i, _vp, _vp, _sz, _sz, _i, _vp],
     "fp_hip_op_expectation": [_vp, _i, _vp, _vp, _sz, _sz, _i, _vp],
+    "fp_hip_op_expectation_plan": [_vp, _i, _sz, _sz, _vp],
     "fp_hip_strings_expectation_matrix": [_vp, _i, _vp, _vp, _sz, _sz, _vp],
     "fp_hip_summed_expectation": [_vp, _i, _i, _vp, _vp, _sz, _sz, _i, _vp],
     "fp_hip_summed_apply": [_vp, _i, _vp, _vp, _sz, _sz, _i, _vp],
     "fp_hip_summed_apply_weighted": [_vp, _i, _vp, _vp, _vp, _sz, _sz, _i, _vp],
     "fp_hip_gemm": [_i, _sz, _sz, _sz, _vp, _vp, _vp, _vp],
 }
+
+
+FP_EV_STREAM, FP_EV_TILED = 0, 1
+
+
+class EvPlan(ctypes.Structure):
+    """fp_hip_ev_plan"""
+    _fields_ = [("kernel", ctypes.c_int), ("n_groups", ctypes.c_uint32), ("tile_rows_log2", ctypes.c_uint32),
+                ("states_per_block", ctypes.c_uint32), ("hbm_bytes", ctypes.c_uint64)]
+
+    def as_dict(self) -> dict:
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+
+def op_expectation_plan(ops, dtype: int, dim: int, n_states: int) -> dict:
+    """Which kernel fp_hip_op_expectation runs for these shapes, and its HBM bytes."""
+    p = EvPlan()
+    call("fp_hip_op_expectation_plan", ops.handle, dtype, dim, n_states, ctypes.byref(p))
+    return p.as_dict()
 
 
 class BatchSchedule(ctypes.Structure):

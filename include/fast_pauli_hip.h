@@ -163,6 +163,29 @@ fp_status fp_hip_op_apply(fp_hip_ops const *ops, int dtype, void const *states, 
 fp_status fp_hip_op_expectation(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim,
                                 size_t n_states, int accumulate, void *stream);
 
+/* How fp_hip_op_expectation evaluates these shapes.  No counterpart in the reference:
+ * diagnostics and roofline accounting for callers that measure the kernel.
+ *   kernel 0 (FP_EV_STREAM): every string reads the whole state once.
+ *   kernel 1 (FP_EV_TILED):  strings are grouped so that the x masks of a group span a subspace
+ *     of dimension tile_rows_log2 (with its low bits); each coset of it is a tile of
+ *     2^tile_rows_log2 rows x states_per_block states, loaded once into LDS per group.
+ * hbm_bytes: state bytes the chosen kernel reads per call, (strings or groups) x 2^n x S x B. */
+enum
+{
+    FP_EV_STREAM = 0,
+    FP_EV_TILED = 1
+};
+typedef struct fp_hip_ev_plan
+{
+    int kernel;
+    uint32_t n_groups;
+    uint32_t tile_rows_log2;
+    uint32_t states_per_block;
+    uint64_t hbm_bytes;
+} fp_hip_ev_plan;
+fp_status fp_hip_op_expectation_plan(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_states,
+                                     fp_hip_ev_plan *plan);
+
 /* Per-string expectation values without coefficients, as a REAL (n_strings x n_states) matrix
  * ev(j,t) = <psi_t|P_j|psi_t> (FP64 for FP_C128, FP32 for FP_C64).  Pauli strings are Hermitian,
  * so the imaginary part is exactly zero.  Stage 1 of SummedPauliOp::expectation_value

@@ -180,7 +180,7 @@ def _structured_codes(kind, N, n, seed):
     raise ValueError(kind)
 
 
-@pytest.mark.parametrize("kernel", ["stream", "tiled", "auto"])
+@pytest.mark.parametrize("kernel", ["stream", "tiled", "tiled_wide", "auto"])
 @pytest.mark.parametrize("dtype", [C128, C64])
 @pytest.mark.parametrize("kind,n,N,S", [
     ("random", 2, 5, 1), ("random", 8, 40, 3), ("random", 11, 64, 16), ("random", 13, 200, 7),
@@ -193,7 +193,9 @@ def test_op_expectation_kernels(hip, monkeypatch, kernel, dtype, kind, n, N, S):
     ragged state counts; same bars as above.  Repeated calls are identical."""
     H.say(f"op_expectation kernel={kernel} {kind} n={n} N={N} S={S} dtype={dtype}")
     if kernel != "auto":
-        monkeypatch.setenv("FAST_PAULI_EV_KERNEL", kernel)
+        monkeypatch.setenv("FAST_PAULI_EV_KERNEL", kernel.split("_")[0])
+    if kernel == "tiled_wide":  # 64-bit element offsets, normally used from 2^32 elements up
+        monkeypatch.setenv("FAST_PAULI_TILE_WIDE", "1")
     cdt = H.CDT[dtype]
     codes = _structured_codes(kind, N, n, seed=5 * n + N)
     strings = syn.codes_to_strings(codes)
@@ -210,6 +212,29 @@ def test_op_expectation_kernels(hip, monkeypatch, kernel, dtype, kind, n, N, S):
     base = syn.random_coeffs(S, seed=98).astype(cdt)
     got2 = H.op_expectation(ops, psi, dtype, out0=base)
     assert H.rel_err(got2 - base, exp) <= 10 * tol
+
+
+def test_op_expectation_plan(hip, monkeypatch):
+    """fp_hip_op_expectation_plan: random strings group 8 per tile at 16
+    complex128 states (256-byte rows, 2^8-row tiles); the bytes follow the
+    groups; a forced streaming kernel reports one state read per string."""
+    H.say("op_expectation plan")
+    n, N, S = 16, 256, 16
+    ops = _abi.Ops(syn.random_codes(N, n, seed=4), syn.random_coeffs(N, seed=4))
+    p = _abi.op_expectation_plan(ops, C128, 1 << n, S)
+    assert p["kernel"] == _abi.FP_EV_TILED
+    assert p["tile_rows_log2"] == 8 and p["states_per_block"] == 16
+    assert N // 8 <= p["n_groups"] <= N // 4
+    assert p["hbm_bytes"] == p["n_groups"] * (1 << n) * S * 16
+    p64 = _abi.op_expectation_plan(ops, C64, 1 << n, S)
+    assert p64["tile_rows_log2"] == 9 and p64["n_groups"] <= p["n_groups"]
+    single = _abi.Ops(syn.random_codes(1, n, seed=5))
+    assert _abi.op_expectation_plan(single, C128, 1 << n, S)["kernel"] == _abi.FP_EV_STREAM
+    monkeypatch.setenv("FAST_PAULI_EV_KERNEL", "stream")
+    q = _abi.op_expectation_plan(ops, C128, 1 << n, S)
+    assert q["kernel"] == _abi.FP_EV_STREAM and q["hbm_bytes"] == N * (1 << n) * S * 16
+    with pytest.raises(ValueError):
+        _abi.op_expectation_plan(ops, C128, 1 << (n - 1), S)
 
 
 def test_expectation_known_answers_and_invariants(hip):
@@ -326,13 +351,18 @@ def test_shape_errors(hip):
         _abi.Ops(np.array([[0, 4]], np.uint8))
 
 
-@pytest.mark.parametrize("dtype,n,S", [(C64, 6, 3), (C64, 13, 40), (C64, 14, 17), (C128, 13, 5),
-                                       (C128, 9, 1100), (C64, 15, 4)])
-def test_strings_expectation_matrix_all_row_modes(hip, dtype, n, S):
+@pytest.mark.parametrize("dtype,n,S,kernel", [
+    (C64, 6, 3, "auto"), (C64, 13, 40, "auto"), (C64, 14, 17, "auto"), (C128, 13, 5, "auto"),
+    (C128, 9, 1100, "auto"), (C64, 15, 4, "auto"), (C128, 14, 9, "auto"), (C128, 14, 9, "stream"),
+    (C128, 15, 1, "auto"), (C128, 16, 3, "auto"), (C64, 17, 2, "auto"), (C64, 15, 4, "stream")])
+def test_strings_expectation_matrix_all_row_modes(hip, monkeypatch, dtype, n, S, kernel):
     """Strings with x = 0, with all X/Y letters on the low 6 row bits, and with
-    higher X/Y letters (the three row enumerations of the LDS kernel; n = 15
-    takes the streaming kernel)."""
-    H.say(f"ev matrix modes n={n} S={S} dtype={dtype}")
+    higher X/Y letters (the three row enumerations of the LDS kernels).  Whole
+    columns fit in LDS up to n = 13 (complex128) / 14 (complex64); above that
+    the coset kernel runs, or the streaming kernel when forced."""
+    H.say(f"ev matrix modes n={n} S={S} dtype={dtype} kernel={kernel}")
+    if kernel != "auto":
+        monkeypatch.setenv("FAST_PAULI_EV_MATRIX_KERNEL", kernel)
     rng = np.random.default_rng(n * 100 + S)
     strings = []
     for _ in range(40):  # diagonal strings
