@@ -182,7 +182,7 @@ __global__ __launch_bounds__(TKB * 8) void k_op_expect_tiled(typename fpk::VecOf
     // group has fewer chunks than waves.  Within a wave, lane = (p_first, t):
     // 2^r = 64 / S_blk leader rows per pass.
     constexpr uint32_t W = kThreads / 64;
-    uint32_t const wave = tid >> 6, lane = tid & 63u;
+    uint32_t const wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63u;
     uint32_t const r = 6u - lsb;
     uint32_t const p_first = lane >> lsb;
     uint32_t split = 1;
@@ -217,10 +217,14 @@ __global__ __launch_bounds__(TKB * 8) void k_op_expect_tiled(typename fpk::VecOf
             uint32_t const it_per = (n_it + split - 1) / split;
             uint32_t const it_begin = (u % split) * it_per;
             uint32_t const it_end = it_begin + it_per < n_it ? it_begin + it_per : n_it;
+            // Dead lanes (only when a tile has fewer leader rows than a pass) read a
+            // valid row and drop their sums; the loops stay wave-uniform, so the pass
+            // index, addresses' uniform part and signs live in scalar registers.
             bool const lane_live = p_first < n_rows;
+            uint32_t const p_lane = lane_live ? p_first : 0u;
             // insert_zero_bit(v, h) = v + (v & hmask) for A != 0
             uint32_t const hmask = A ? ~((1u << h) - 1u) : 0u;
-            uint32_t const a_first = p_first + (p_first & hmask);
+            uint32_t const a_first = p_lane + (p_lane & hmask);
             uint32_t const off0 = ((a_first << lsb) | t) * kEsz;
             uint32_t const pxor = (A << lsb) * kEsz;
             uint32_t zt[kChunk];
@@ -238,12 +242,15 @@ __global__ __launch_bounds__(TKB * 8) void k_op_expect_tiled(typename fpk::VecOf
                     imag[s] = strs[ch.first + s].imag != 0;
                 }
             }
-            if (lane_live && it_begin < it_end)
+            auto uniform_off = [&](uint32_t it) {
+                uint32_t const pit = it << r;
+                return ((pit + (pit & hmask)) << lsb) * kEsz;
+            };
+            if (ch.count == 1)
             {
-                if (ch.count == 1)
-                {
-                    // one string: only the needed part of w, sign by a scalar +-1
-                    bool const im = imag[0];
+                // one string: only the needed half of w; sign as a scalar +-1
+                auto run = [&](auto im_tag) {
+                    constexpr bool IM = decltype(im_tag)::value;
                     uint32_t const zr = zt[0] >> r;
                     Acc p0 = Acc(0);
                     for (uint32_t i0 = it_begin; i0 < it_end; i0 += U)
@@ -253,8 +260,7 @@ __global__ __launch_bounds__(TKB * 8) void k_op_expect_tiled(typename fpk::VecOf
                         for (int uu = 0; uu < U; ++uu)
                         {
                             uint32_t const it = i0 + uu < it_end ? i0 + uu : i0;
-                            uint32_t const pit = it << r;
-                            uint32_t const addr = off0 ^ (((pit + (pit & hmask)) << lsb) * kEsz);
+                            uint32_t const addr = off0 ^ uniform_off(it);
                             va[uu] = *reinterpret_cast<V const *>(lds + addr);
                             vb[uu] = *reinterpret_cast<V const *>(lds + (addr ^ pxor));
                         }
@@ -263,53 +269,62 @@ __global__ __launch_bounds__(TKB * 8) void k_op_expect_tiled(typename fpk::VecOf
                         {
                             if (i0 + uu < it_end)
                             {
-                                uint32_t const it = i0 + uu;
+                                Acc const sg = (__popc((i0 + uu) & zr) & 1) ? Acc(-1) : Acc(1);
                                 Acc const ar = Acc(va[uu][0]), ai = Acc(va[uu][1]);
                                 Acc const br = Acc(vb[uu][0]), bi = Acc(vb[uu][1]);
-                                Acc const gv = im ? (ar * bi - ai * br) : (ar * br + ai * bi);
-                                p0 = (__popc(it & zr) & 1) ? p0 - gv : p0 + gv;
+                                Acc const gv = IM ? fma(ar, bi, -(ai * br)) : fma(ar, br, ai * bi);
+                                p0 = fma(sg, gv, p0);
                             }
                         }
                     }
                     part[0] = p0;
-                }
+                };
+                if (imag[0])
+                    run(std::true_type{});
                 else
+                    run(std::false_type{});
+            }
+            else
+            {
+                for (uint32_t i0 = it_begin; i0 < it_end; i0 += U)
                 {
-                    for (uint32_t i0 = it_begin; i0 < it_end; i0 += U)
+                    V va[U], vb[U];
+#pragma unroll
+                    for (int uu = 0; uu < U; ++uu)
                     {
-                        V va[U], vb[U];
+                        uint32_t const it = i0 + uu < it_end ? i0 + uu : i0;
+                        uint32_t const addr = off0 ^ uniform_off(it);
+                        va[uu] = *reinterpret_cast<V const *>(lds + addr);
+                        vb[uu] = *reinterpret_cast<V const *>(lds + (addr ^ pxor));
+                    }
 #pragma unroll
-                        for (int uu = 0; uu < U; ++uu)
+                    for (int uu = 0; uu < U; ++uu)
+                    {
+                        if (i0 + uu < it_end)
                         {
-                            uint32_t const it = i0 + uu < it_end ? i0 + uu : i0;
-                            uint32_t const pit = it << r;
-                            uint32_t const addr = off0 ^ (((pit + (pit & hmask)) << lsb) * kEsz);
-                            va[uu] = *reinterpret_cast<V const *>(lds + addr);
-                            vb[uu] = *reinterpret_cast<V const *>(lds + (addr ^ pxor));
-                        }
+                            uint32_t const it = i0 + uu;
+                            Acc const ar = Acc(va[uu][0]), ai = Acc(va[uu][1]);
+                            Acc const br = Acc(vb[uu][0]), bi = Acc(vb[uu][1]);
+                            Acc const gre = fma(ar, br, ai * bi);
+                            Acc const gim = fma(ar, bi, -(ai * br));
 #pragma unroll
-                        for (int uu = 0; uu < U; ++uu)
-                        {
-                            if (i0 + uu < it_end)
+                            for (int s = 0; s < kChunk; ++s)
                             {
-                                uint32_t const it = i0 + uu;
-                                Acc const ar = Acc(va[uu][0]), ai = Acc(va[uu][1]);
-                                Acc const br = Acc(vb[uu][0]), bi = Acc(vb[uu][1]);
-                                Acc const gre = ar * br + ai * bi;
-                                Acc const gim = ar * bi - ai * br;
-#pragma unroll
-                                for (int s = 0; s < kChunk; ++s)
+                                if (s < int(ch.count))
                                 {
-                                    if (s < int(ch.count))
-                                    {
-                                        Acc const gv = imag[s] ? gim : gre;
-                                        part[s] = (__popc(it & (zt[s] >> r)) & 1) ? part[s] - gv : part[s] + gv;
-                                    }
+                                    Acc const sg = (__popc(it & (zt[s] >> r)) & 1) ? Acc(-1) : Acc(1);
+                                    part[s] = fma(sg, imag[s] ? gim : gre, part[s]);
                                 }
                             }
                         }
                     }
                 }
+            }
+            if (!lane_live)
+            {
+#pragma unroll
+                for (int s = 0; s < kChunk; ++s)
+                    part[s] = Acc(0);
             }
 #pragma unroll
             for (int s = 0; s < kChunk; ++s)
