@@ -123,6 +123,7 @@ __global__ __launch_bounds__(TKB * 8) void k_op_expect_tiled(typename fpk::VecOf
     constexpr uint32_t EPT = kElems / kThreads;
     constexpr uint32_t kEsz = sizeof(V);
     constexpr int U = 4;
+    constexpr int U1 = sizeof(V) == 16 ? 4 : 8; // passes per group in the one-string fast path
     extern __shared__ __align__(16) unsigned char smem[];
     V *tile = reinterpret_cast<V *>(smem);
 
@@ -246,9 +247,59 @@ __global__ __launch_bounds__(TKB * 8) void k_op_expect_tiled(typename fpk::VecOf
                 uint32_t const pit = it << r;
                 return ((pit + (pit & hmask)) << lsb) * kEsz;
             };
-            if (ch.count == 1)
+            if (ch.count == 1 && (it_begin % U1) == 0 && ((it_end - it_begin) % U1) == 0)
             {
-                // one string: only the needed half of w; sign as a scalar +-1
+                // One string, whole groups of U1 passes: pass i0 + uu (i0 a multiple of
+                // U1) has uniform address term uoff(i0) ^ uoff(uu) and row sign
+                // parity(i0 & zr) ^ parity(uu & zr).  Per-uu lane bases and per-uu
+                // accumulators carry the uu parts, so each group of passes costs one
+                // scalar offset and one scalar sign.
+                auto run = [&](auto im_tag) {
+                    constexpr bool IM = decltype(im_tag)::value;
+                    uint32_t const zr = zt[0] >> r;
+                    uint32_t lb[U1];
+                    Acc pu[U1];
+#pragma unroll
+                    for (int uu = 0; uu < U1; ++uu)
+                    {
+                        lb[uu] = off0 ^ uniform_off(uint32_t(uu));
+                        pu[uu] = Acc(0);
+                    }
+                    for (uint32_t i0 = it_begin; i0 < it_end; i0 += U1)
+                    {
+                        uint32_t const ub = uniform_off(i0);
+                        Acc const sg = (__popc(i0 & zr) & 1) ? Acc(-1) : Acc(1);
+                        V va[U1], vb[U1];
+#pragma unroll
+                        for (int uu = 0; uu < U1; ++uu)
+                        {
+                            uint32_t const addr = lb[uu] ^ ub;
+                            va[uu] = *reinterpret_cast<V const *>(lds + addr);
+                            vb[uu] = *reinterpret_cast<V const *>(lds + (addr ^ pxor));
+                        }
+#pragma unroll
+                        for (int uu = 0; uu < U1; ++uu)
+                        {
+                            Acc const ar = Acc(va[uu][0]), ai = Acc(va[uu][1]);
+                            Acc const br = Acc(vb[uu][0]), bi = Acc(vb[uu][1]);
+                            Acc const gv = IM ? fma(ar, bi, -(ai * br)) : fma(ar, br, ai * bi);
+                            pu[uu] = fma(sg, gv, pu[uu]);
+                        }
+                    }
+                    Acc p0 = Acc(0);
+#pragma unroll
+                    for (int uu = 0; uu < U1; ++uu)
+                        p0 += (__popc(uint32_t(uu) & zr) & 1) ? -pu[uu] : pu[uu];
+                    part[0] = p0;
+                };
+                if (imag[0])
+                    run(std::true_type{});
+                else
+                    run(std::false_type{});
+            }
+            else if (ch.count == 1)
+            {
+                // one string, ragged pass range: only the needed half of w; scalar +-1 sign
                 auto run = [&](auto im_tag) {
                     constexpr bool IM = decltype(im_tag)::value;
                     uint32_t const zr = zt[0] >> r;
