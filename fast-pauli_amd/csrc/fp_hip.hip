@@ -298,7 +298,7 @@ fp_status ops_assign(fp_hip_ops *o, size_t n_qubits, size_t n_strings, uint8_t c
     for (size_t i = 0; i < n_qubits * n_strings; ++i)
         if (codes[i] > 3)
             return fail(FP_ERR_INVALID_ARGUMENT, "Pauli code must be 0, 1, 2, or 3 (got %u)", unsigned(codes[i]));
-    fprt::release_tiles(o); // string groups belong to the previous strings
+    fprt::invalidate_tiles(o); // string groups belong to the previous strings
     ArenaLayout const L = arena_layout(n_strings);
     if (o->upload_done)
         FP_HIP(hipEventSynchronize(o->upload_done)); // the previous upload may still read h_arena
@@ -1063,7 +1063,7 @@ fp_status fp_hip_ops_set_coeffs(fp_hip_ops *ops, double const *coeffs)
     if (!coeffs)
         return fail(FP_ERR_INVALID_ARGUMENT, "coeffs is NULL");
     (void)hipDeviceSynchronize();
-    fprt::release_tiles(ops); // the tiled expectation tables hold the scaled coefficients
+    fprt::invalidate_tiles(ops); // the tiled expectation tables hold the scaled coefficients
     std::memcpy(ops->coeffs.data(), coeffs, 2 * ops->n_strings * sizeof(double));
     ArenaLayout const L = arena_layout(ops->n_strings);
     fill_coeff_tables(ops, L);

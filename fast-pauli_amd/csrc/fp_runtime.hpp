@@ -56,13 +56,21 @@ enum
     kTilesEvMatrix = 1  // fpl::EvChunk / fpl::EvStr records (per-string EV matrix on cosets)
 };
 
-// Device tables of the string groups of one kind and geometry (k, dh), cached
-// per operator set.
+// String-group tables of one kind and geometry (k, dh), cached per operator
+// set: a host image (groups | B tables | chunks | strings) and its device copy.
+// Re-targeting the operator set invalidates them but keeps both buffers for
+// reuse, so re-targeted sets (the serving loop) neither allocate nor free.
 struct TileTables
 {
     int kind = 0;
     uint32_t k = 0, dh = 0, n_groups = 0, max_chunks = 0;
+    bool valid = false;     // built for the current strings and coefficients
+    bool on_device = false; // the device copy matches the host image
+    std::vector<unsigned char> h;
+    size_t o_btab = 0, o_chunks = 0, o_strs = 0;
     void *d_mem = nullptr;
+    size_t cap = 0;
+    hipEvent_t uploaded = nullptr; // completes when the last upload has read `h`
     fpt::TileGroup *groups = nullptr;
     uint64_t *btab = nullptr;
     void *chunks = nullptr;
@@ -70,7 +78,9 @@ struct TileTables
 };
 
 TileGeom tile_geom(int dtype, size_t n_qubits, size_t n_states);
-fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, TileTables const **out);
+fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, TileTables **out);
+fp_status tile_device(fp_hip_ops const *ops, TileTables *t, hipStream_t stream);
+void invalidate_tiles(fp_hip_ops *ops);
 // Per-string EV matrix over cosets: for columns that do not fit in LDS whole.
 bool ev_coset_eligible(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_states);
 size_t ev_coset_scratch_bytes(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_states);

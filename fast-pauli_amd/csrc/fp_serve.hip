@@ -362,6 +362,11 @@ void admit(fp_hip_server *s, Slot &sl)
     s->peak_in_use = std::max(s->peak_in_use, s->pool->in_use());
     fp_status st = fp_hip_batch_clear(sl.batch);
     double const t0 = now_s();
+    // Expectation requests whose strings group into shared tiles run through the
+    // tiled kernel (one state read per string group) after the batch, on the same
+    // stream; everything else goes into the persistent batch launch.
+    std::vector<Request *> tiled;
+    size_t n_batched = 0;
     for (Request *r : take)
     {
         r->phase = kRunning;
@@ -376,13 +381,29 @@ void admit(fp_hip_server *s, Slot &sl)
         if (st == FP_OK && !r->spec.states)
             st = fp_hip_fill_states(r->spec.dtype, r->states.ptr, dim, r->spec.n_states, r->spec.state_seed, 1,
                                     sl.stream);
-        if (st == FP_OK)
+        bool use_tiled = false;
+        if (st == FP_OK && r->spec.kind == FP_BATCH_EXPECT)
+            st = fprt::choose_op_expect_tiled(r->ops, r->spec.dtype, r->spec.n_states, &use_tiled);
+        if (st == FP_OK && use_tiled)
+            tiled.push_back(r);
+        else if (st == FP_OK)
+        {
             st = fp_hip_batch_add(sl.batch, r->spec.kind, r->ops, r->spec.dtype, in, r->out.ptr, dim,
                                   r->spec.n_states, 0);
+            ++n_batched;
+        }
         sl.reqs.push_back(r);
     }
-    if (st == FP_OK)
+    if (st == FP_OK && n_batched > 0)
         st = fp_hip_batch_launch(sl.batch, sl.stream);
+    for (Request *r : tiled)
+    {
+        if (st != FP_OK)
+            break;
+        void const *in = r->spec.states ? r->spec.states : r->states.ptr;
+        st = fp_hip_op_expectation(r->ops, r->spec.dtype, in, r->out.ptr, size_t(1) << r->spec.n_qubits,
+                                   r->spec.n_states, 0, sl.stream);
+    }
     if (st == FP_OK && hipEventRecord(sl.done, sl.stream) != hipSuccess)
         st = FP_ERR_RUNTIME;
     if (st != FP_OK)

@@ -234,11 +234,11 @@ fpt::TileGroup group_header(HostGroup const &hg, std::vector<uint64_t> &btab)
 } // namespace
 
 // Builds (or returns the cached) tables of `ops` of one kind for geometry (k, dh).
-fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, TileTables const **out)
+fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, TileTables **out)
 {
     std::lock_guard<std::mutex> lock(ops->tiles_mu);
-    for (auto const &t : ops->tiles)
-        if (t.kind == kind && t.k == k && t.dh == dh)
+    for (auto &t : ops->tiles)
+        if (t.valid && t.kind == kind && t.k == k && t.dh == dh)
         {
             *out = &t;
             return FP_OK;
@@ -342,46 +342,103 @@ fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, 
     for (auto const &g : groups)
         max_chunks = std::max(max_chunks, g.n_chunks);
 
-    TileTables t{};
-    t.kind = kind;
-    t.k = k;
-    t.dh = dh;
-    t.n_groups = uint32_t(groups.size());
-    t.max_chunks = max_chunks;
-    auto up = [](size_t v) { return (v + 255) / 256 * 256; };
-    size_t const o_groups = 0;
-    size_t const o_btab = o_groups + up(groups.size() * sizeof(fpt::TileGroup));
-    size_t const o_chunks = o_btab + up(btab.size() * sizeof(uint64_t));
-    size_t const o_strs = o_chunks + up(chunks.size());
-    size_t const total = o_strs + up(strs.size());
-    std::vector<unsigned char> h(total, 0);
-    std::memcpy(h.data() + o_groups, groups.data(), groups.size() * sizeof(fpt::TileGroup));
-    std::memcpy(h.data() + o_btab, btab.data(), btab.size() * sizeof(uint64_t));
-    std::memcpy(h.data() + o_chunks, chunks.data(), chunks.size());
-    std::memcpy(h.data() + o_strs, strs.data(), strs.size());
-    FP_HIP(hipMalloc(&t.d_mem, total));
-    hipError_t const e = hipMemcpy(t.d_mem, h.data(), total, hipMemcpyHostToDevice);
-    if (e != hipSuccess)
+    // reuse an invalidated entry (its host image and device memory) when there is one
+    TileTables *t = nullptr;
+    for (auto &e : ops->tiles)
+        if (!e.valid)
+        {
+            t = &e;
+            break;
+        }
+    if (!t)
     {
-        (void)hipFree(t.d_mem);
-        return fail(FP_ERR_RUNTIME, "tile table upload failed: %s", hipGetErrorString(e));
+        ops->tiles.emplace_back();
+        t = &ops->tiles.back();
     }
-    auto *base = static_cast<unsigned char *>(t.d_mem);
-    t.groups = reinterpret_cast<fpt::TileGroup *>(base + o_groups);
-    t.btab = reinterpret_cast<uint64_t *>(base + o_btab);
-    t.chunks = base + o_chunks;
-    t.strs = base + o_strs;
-    ops->tiles.push_back(t);
-    *out = &ops->tiles.back();
+    if (t->uploaded)
+    {
+        // the previous upload may still be reading the host image
+        hipError_t const e = hipEventSynchronize(t->uploaded);
+        if (e != hipSuccess)
+            return fail(FP_ERR_RUNTIME, "tile table event wait failed: %s", hipGetErrorString(e));
+    }
+    t->kind = kind;
+    t->k = k;
+    t->dh = dh;
+    t->n_groups = uint32_t(groups.size());
+    t->max_chunks = max_chunks;
+    auto up = [](size_t v) { return (v + 255) / 256 * 256; };
+    t->o_btab = up(groups.size() * sizeof(fpt::TileGroup));
+    t->o_chunks = t->o_btab + up(btab.size() * sizeof(uint64_t));
+    t->o_strs = t->o_chunks + up(chunks.size());
+    size_t const total = t->o_strs + up(strs.size());
+    t->h.assign(total, 0);
+    std::memcpy(t->h.data(), groups.data(), groups.size() * sizeof(fpt::TileGroup));
+    std::memcpy(t->h.data() + t->o_btab, btab.data(), btab.size() * sizeof(uint64_t));
+    std::memcpy(t->h.data() + t->o_chunks, chunks.data(), chunks.size());
+    std::memcpy(t->h.data() + t->o_strs, strs.data(), strs.size());
+    t->on_device = false;
+    t->valid = true;
+    *out = t;
     return FP_OK;
+}
+
+// Makes the device copy of `t` current for kernels on `stream`: reuses its
+// device memory when large enough, uploads asynchronously on `stream` and
+// records an event that guards the host image.
+fp_status tile_device(fp_hip_ops const *ops, TileTables *t, hipStream_t stream)
+{
+    std::lock_guard<std::mutex> lock(ops->tiles_mu);
+    if (t->on_device)
+        return FP_OK;
+    size_t const total = t->h.size();
+    if (t->cap < total)
+    {
+        if (t->d_mem)
+        {
+            FP_HIP(hipFree(t->d_mem));
+            t->d_mem = nullptr;
+            t->cap = 0;
+        }
+        FP_HIP(hipMalloc(&t->d_mem, total));
+        t->cap = total;
+    }
+    if (!t->uploaded)
+        FP_HIP(hipEventCreateWithFlags(&t->uploaded, hipEventDisableTiming));
+    FP_HIP(hipMemcpyAsync(t->d_mem, t->h.data(), total, hipMemcpyHostToDevice, stream));
+    FP_HIP(hipEventRecord(t->uploaded, stream));
+    auto *base = static_cast<unsigned char *>(t->d_mem);
+    t->groups = reinterpret_cast<fpt::TileGroup *>(base);
+    t->btab = reinterpret_cast<uint64_t *>(base + t->o_btab);
+    t->chunks = base + t->o_chunks;
+    t->strs = base + t->o_strs;
+    t->on_device = true;
+    return FP_OK;
+}
+
+void invalidate_tiles(fp_hip_ops *ops)
+{
+    std::lock_guard<std::mutex> lock(ops->tiles_mu);
+    for (auto &t : ops->tiles)
+    {
+        t.valid = false;
+        t.on_device = false;
+    }
 }
 
 void release_tiles(fp_hip_ops *ops)
 {
     std::lock_guard<std::mutex> lock(ops->tiles_mu);
     for (auto &t : ops->tiles)
+    {
+        if (t.uploaded)
+        {
+            (void)hipEventSynchronize(t.uploaded);
+            (void)hipEventDestroy(t.uploaded);
+        }
         if (t.d_mem)
             (void)hipFree(t.d_mem);
+    }
     ops->tiles.clear();
 }
 
@@ -397,7 +454,7 @@ fp_status choose_op_expect_tiled(fp_hip_ops const *ops, int dtype, size_t n_stat
     if (!forced && ops->n_strings < 2)
         return FP_OK;
     TileGeom const geo = tile_geom(dtype, ops->n_qubits, n_states);
-    TileTables const *tt = nullptr;
+    TileTables *tt = nullptr;
     fp_status const st = tile_tables(ops, kTilesOpExpect, geo.k, geo.dh, &tt);
     if (st != FP_OK)
         return st;
@@ -409,9 +466,11 @@ fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *stat
                               size_t n_states, int accumulate, hipStream_t st)
 {
     TileGeom const geo = tile_geom(dtype, ops->n_qubits, n_states);
-    TileTables const *tt = nullptr;
+    TileTables *tt = nullptr;
     fp_status s = tile_tables(ops, kTilesOpExpect, geo.k, geo.dh, &tt);
     if (s != FP_OK)
+        return s;
+    if ((s = tile_device(ops, tt, st)) != FP_OK)
         return s;
     uint64_t const n_cosets = uint64_t(dim) >> geo.d;
     uint32_t const chunks_y = uint32_t((n_states + (size_t(1) << geo.lsb) - 1) >> geo.lsb);
@@ -545,7 +604,7 @@ size_t ev_coset_scratch_bytes(fp_hip_ops const *ops, int dtype, size_t dim, size
 }
 
 template <class T>
-fp_status launch_ev_coset(fp_hip_ops const *ops, TileTables const *tt, CosetGeom const &g, void const *states,
+fp_status launch_ev_coset(fp_hip_ops const *ops, TileTables *tt, CosetGeom const &g, void const *states,
                           void *ev, size_t dim, size_t n_states, void *scratch, hipStream_t st)
 {
     typedef typename fpk::VecOf<T, 1>::type V;
@@ -585,9 +644,11 @@ fp_status run_ev_coset(fp_hip_ops const *ops, int dtype, void const *states, voi
                        void *scratch, hipStream_t st)
 {
     CosetGeom const g = coset_geom(dtype, ops->n_qubits);
-    TileTables const *tt = nullptr;
-    fp_status const s = tile_tables(ops, kTilesEvMatrix, g.k, g.d - g.k, &tt);
+    TileTables *tt = nullptr;
+    fp_status s = tile_tables(ops, kTilesEvMatrix, g.k, g.d - g.k, &tt);
     if (s != FP_OK)
+        return s;
+    if ((s = tile_device(ops, tt, st)) != FP_OK)
         return s;
     if (tt->n_groups * g.n_cos > 65535u)
         return fail(FP_ERR_INVALID_ARGUMENT, "too many string groups for the coset kernel");
@@ -622,7 +683,7 @@ extern "C" fp_status fp_hip_op_expectation_plan(fp_hip_ops const *ops, int dtype
         return FP_OK;
     }
     fprt::TileGeom const geo = fprt::tile_geom(dtype, ops->n_qubits, n_states);
-    fprt::TileTables const *tt = nullptr;
+    fprt::TileTables *tt = nullptr;
     if ((st = fprt::tile_tables(ops, fprt::kTilesOpExpect, geo.k, geo.dh, &tt)) != FP_OK)
         return st;
     plan->kernel = FP_EV_TILED;

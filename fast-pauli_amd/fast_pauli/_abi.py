@@ -392,6 +392,24 @@ class Ops:
     def dim(self) -> int:
         return 1 << self.n_qubits
 
+    def assign(self, codes: np.ndarray, coeffs=None, stream=None) -> None:
+        """Re-targets this set to new strings in place (fp_hip_ops_assign); the
+        device tables are rewritten on `stream` and reallocated only if they grow."""
+        codes = np.ascontiguousarray(codes, dtype=np.uint8)
+        if codes.ndim != 2:
+            raise ValueError("codes must be (n_strings, n_qubits)")
+        c_ptr = None
+        cf = None
+        if coeffs is not None:
+            cf = np.ascontiguousarray(coeffs, dtype=np.complex128).reshape(-1)
+            if cf.size != codes.shape[0]:
+                raise ValueError("coeffs and pauli_strings must have the same size")
+            c_ptr = cf.ctypes.data
+        call("fp_hip_ops_assign", self.handle, codes.shape[1], codes.shape[0], codes.ctypes.data, c_ptr, stream)
+        self.n_strings, self.n_qubits = codes.shape
+        self._codes, self._coeffs = codes, cf
+        self.n_operators = 0
+
     def set_coeffs(self, coeffs) -> None:
         c = np.ascontiguousarray(coeffs, dtype=np.complex128).reshape(-1)
         if c.size != self.n_strings:

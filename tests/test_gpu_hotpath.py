@@ -237,6 +237,33 @@ def test_op_expectation_plan(hip, monkeypatch):
         _abi.op_expectation_plan(ops, C128, 1 << (n - 1), S)
 
 
+@pytest.mark.parametrize("dtype", [C128, C64])
+def test_tiled_tables_follow_assign_and_coeffs(hip, monkeypatch, dtype):
+    """The tiled kernel's cached string groups are rebuilt after
+    fp_hip_ops_assign (new strings, device memory reused) and after
+    fp_hip_ops_set_coeffs (new scales): results equal a fresh operator set."""
+    H.say(f"tiled tables after assign / set_coeffs dtype={dtype}")
+    monkeypatch.setenv("FAST_PAULI_EV_KERNEL", "tiled")
+    n, S = 12, 8
+    cdt = H.CDT[dtype]
+    psi = syn.random_states(1 << n, S, seed=31).astype(cdt)
+    ops = _abi.Ops(syn.random_codes(64, n, seed=31), syn.random_coeffs(64, seed=31))
+    H.op_expectation(ops, psi, dtype)
+    for N, seed in ((100, 32), (40, 33), (100, 34)):
+        codes = syn.random_codes(N, n, seed=seed)
+        coeffs = syn.random_coeffs(N, seed=seed)
+        ops.assign(codes, coeffs)
+        _abi.synchronize()
+        got = H.op_expectation(ops, psi, dtype)
+        ref = H.op_expectation(_abi.Ops(codes, coeffs), psi, dtype)
+        assert np.array_equal(got, ref), (N, seed)
+    new = syn.random_coeffs(100, seed=35)
+    ops.set_coeffs(new)
+    got = H.op_expectation(ops, psi, dtype)
+    ref = H.op_expectation(_abi.Ops(syn.random_codes(100, n, seed=34), new), psi, dtype)
+    assert np.array_equal(got, ref)
+
+
 def test_expectation_known_answers_and_invariants(hip):
     H.say("expectation known answers")
     a = np.arange(8).astype(np.complex128).reshape(8, 1)
