@@ -207,6 +207,8 @@ void fill_coeff_tables(fp_hip_ops *o, ArenaLayout const &L)
 {
     void *h = o->h_arena.data();
     size_t const N = o->n_strings;
+    o->cf128.resize(2 * N);
+    o->cf64.resize(2 * N);
     for (size_t s = 0; s < N; ++s)
     {
         double ur, ui;
@@ -214,9 +216,13 @@ void fill_coeff_tables(fp_hip_ops *o, ArenaLayout const &L)
         double const cr = o->coeffs[2 * s], ci = o->coeffs[2 * s + 1];
         double *cf128 = at<double>(h, L.cf128);
         mul_axis_unit(cr, ci, ur, ui, cf128[2 * s], cf128[2 * s + 1]);
+        o->cf128[2 * s] = cf128[2 * s];
+        o->cf128[2 * s + 1] = cf128[2 * s + 1];
         double fr, fi;
         mul_axis_unit(double(float(cr)), double(float(ci)), ur, ui, fr, fi);
         at<float>(h, L.cf64)[2 * s] = float(fr);
+        o->cf64[2 * s] = float(fr);
+        o->cf64[2 * s + 1] = float(fi);
         at<float>(h, L.cf64)[2 * s + 1] = float(fi);
         double const k = kappa_of(o->x[s], o->ny[s]);
         at<double>(h, L.kappa)[s] = k;
@@ -1211,6 +1217,11 @@ fp_status fp_hip_op_apply(fp_hip_ops const *ops, int dtype, void const *states, 
     if (states == out)
         return fail(FP_ERR_INVALID_ARGUMENT, "apply cannot run in place (out aliases states)");
     hipStream_t const s = static_cast<hipStream_t>(stream);
+    bool tiled = false;
+    if ((st = fprt::choose_op_apply_tiled(ops, dtype, n_states, &tiled)) != FP_OK)
+        return st;
+    if (tiled)
+        return fprt::run_op_apply_tiled(ops, dtype, states, out, dim, n_states, accumulate, s);
     VecGeom const g = vec_geom(dtype, dim, n_states, states, out);
     uint32_t const N = uint32_t(ops->n_strings);
     if (dtype == FP_C128)

@@ -13,6 +13,7 @@
 
 #include "fast_pauli_hip.h"
 #include "fp_ev_lds.hpp"
+#include "fp_apply_tiled.hpp"
 #include "fp_ev_tiled.hpp"
 #include "fp_kernels.hpp"
 
@@ -53,7 +54,8 @@ struct TileGeom
 enum
 {
     kTilesOpExpect = 0, // fpt::TileChunk / fpt::TileStr records (PauliOp expectation)
-    kTilesEvMatrix = 1  // fpl::EvChunk / fpl::EvStr records (per-string EV matrix on cosets)
+    kTilesEvMatrix = 1, // fpl::EvChunk / fpl::EvStr records (per-string EV matrix on cosets)
+    kTilesOpApply = 2   // fpa::ApplyStr records, groups of consecutive strings (PauliOp apply)
 };
 
 // String-group tables of one kind and geometry (k, dh), cached per operator
@@ -88,6 +90,9 @@ fp_status run_ev_coset(fp_hip_ops const *ops, int dtype, void const *states, voi
                        void *scratch, hipStream_t st);
 void release_tiles(fp_hip_ops *ops);
 fp_status choose_op_expect_tiled(fp_hip_ops const *ops, int dtype, size_t n_states, bool *use_tiled);
+fp_status choose_op_apply_tiled(fp_hip_ops const *ops, int dtype, size_t n_states, bool *use_tiled);
+fp_status run_op_apply_tiled(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim,
+                             size_t n_states, int accumulate, hipStream_t st);
 fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim,
                               size_t n_states, int accumulate, hipStream_t st);
 
@@ -123,6 +128,8 @@ struct fp_hip_ops
     std::vector<uint64_t> x, z;
     std::vector<uint32_t> ny;
     std::vector<double> coeffs;  // complex128, n_strings
+    std::vector<double> cf128;   // folded c_s (-i)^{nY_s}, complex128 (host copy of d_cf128)
+    std::vector<float> cf64;     // the same rounded to complex64 (host copy of d_cf64)
     std::vector<double> matrix;  // complex128, n_strings x n_operators
     // Per-string tables live in one device arena (re-targeted in place by
     // fp_hip_ops_assign, reallocated only when it must grow) mirrored on the host.

@@ -362,9 +362,10 @@ void admit(fp_hip_server *s, Slot &sl)
     s->peak_in_use = std::max(s->peak_in_use, s->pool->in_use());
     fp_status st = fp_hip_batch_clear(sl.batch);
     double const t0 = now_s();
-    // Expectation requests whose strings group into shared tiles run through the
-    // tiled kernel (one state read per string group) after the batch, on the same
-    // stream; everything else goes into the persistent batch launch.
+    // Requests whose strings group into shared coset tiles run through the tiled
+    // kernels (expectation: one state read per group; apply on large states: one
+    // state read and an output read + write per group) after the batch, on the
+    // same stream; everything else goes into the persistent batch launch.
     std::vector<Request *> tiled;
     size_t n_batched = 0;
     for (Request *r : take)
@@ -384,6 +385,8 @@ void admit(fp_hip_server *s, Slot &sl)
         bool use_tiled = false;
         if (st == FP_OK && r->spec.kind == FP_BATCH_EXPECT)
             st = fprt::choose_op_expect_tiled(r->ops, r->spec.dtype, r->spec.n_states, &use_tiled);
+        else if (st == FP_OK)
+            st = fprt::choose_op_apply_tiled(r->ops, r->spec.dtype, r->spec.n_states, &use_tiled);
         if (st == FP_OK && use_tiled)
             tiled.push_back(r);
         else if (st == FP_OK)
@@ -401,8 +404,10 @@ void admit(fp_hip_server *s, Slot &sl)
         if (st != FP_OK)
             break;
         void const *in = r->spec.states ? r->spec.states : r->states.ptr;
-        st = fp_hip_op_expectation(r->ops, r->spec.dtype, in, r->out.ptr, size_t(1) << r->spec.n_qubits,
-                                   r->spec.n_states, 0, sl.stream);
+        size_t const dim = size_t(1) << r->spec.n_qubits;
+        st = r->spec.kind == FP_BATCH_EXPECT
+                 ? fp_hip_op_expectation(r->ops, r->spec.dtype, in, r->out.ptr, dim, r->spec.n_states, 0, sl.stream)
+                 : fp_hip_op_apply(r->ops, r->spec.dtype, in, r->out.ptr, dim, r->spec.n_states, 0, sl.stream);
     }
     if (st == FP_OK && hipEventRecord(sl.done, sl.stream) != hipSuccess)
         st = FP_ERR_RUNTIME;

@@ -133,7 +133,7 @@ struct HostGroup
 
 // Greedy grouping of the strings of `ops` into subspaces of dimension k + dh
 // (the low k bits plus dh reduced basis vectors), in x order.
-fp_status group_strings(fp_hip_ops const *ops, uint32_t k, uint32_t dh, std::vector<HostGroup> &out)
+fp_status group_strings(fp_hip_ops const *ops, uint32_t k, uint32_t dh, bool ordered, std::vector<HostGroup> &out)
 {
     size_t const N = ops->n_strings, n = ops->n_qubits;
     uint64_t const lowmask = (uint64_t(1) << k) - 1;
@@ -142,7 +142,8 @@ fp_status group_strings(fp_hip_ops const *ops, uint32_t k, uint32_t dh, std::vec
     std::vector<uint32_t> order(N);
     for (size_t i = 0; i < N; ++i)
         order[i] = uint32_t(i);
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return ops->x[a] < ops->x[b]; });
+    if (!ordered) // x order groups related strings; apply keeps the operator's order
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return ops->x[a] < ops->x[b]; });
     if (dh == 0)
         for (uint32_t s : order)
             if (ops->x[s] & ~lowmask)
@@ -190,7 +191,8 @@ fp_status group_strings(fp_hip_ops const *ops, uint32_t k, uint32_t dh, std::vec
             g.mem.push_back(LocalStr{s, (alpha << k) | uint32_t(ops->x[s] & lowmask),
                                      (zeta << k) | uint32_t(z & lowmask), zc});
         }
-        std::stable_sort(g.mem.begin(), g.mem.end(), [](LocalStr const &a, LocalStr const &b) { return a.A < b.A; });
+        if (!ordered)
+            std::stable_sort(g.mem.begin(), g.mem.end(), [](LocalStr const &a, LocalStr const &b) { return a.A < b.A; });
         out.push_back(std::move(g));
         span = Span{};
         members.clear();
@@ -244,7 +246,7 @@ fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, 
             return FP_OK;
         }
     std::vector<HostGroup> hgs;
-    fp_status const st = group_strings(ops, k, dh, hgs);
+    fp_status const st = group_strings(ops, k, dh, kind == kTilesOpApply, hgs);
     if (st != FP_OK)
         return st;
     std::vector<fpt::TileGroup> groups;
@@ -255,7 +257,25 @@ fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, 
     {
         fpt::TileGroup g = group_header(hg, btab);
         g.chunk_first = uint32_t(n_chunks);
-        if (kind == kTilesOpExpect)
+        if (kind == kTilesOpApply)
+        {
+            // one record per string, in the operator's order; n_chunks counts them
+            for (LocalStr const &m : hg.mem)
+            {
+                fpa::ApplyStr r{};
+                r.zc = m.zc;
+                r.A = m.A;
+                r.Z = m.Z;
+                r.cr = ops->cf128[2 * m.s];
+                r.ci = ops->cf128[2 * m.s + 1];
+                r.fr = ops->cf64[2 * m.s];
+                r.fi = ops->cf64[2 * m.s + 1];
+                strs.insert(strs.end(), reinterpret_cast<unsigned char const *>(&r),
+                            reinterpret_cast<unsigned char const *>(&r) + sizeof(r));
+                ++n_chunks;
+            }
+        }
+        else if (kind == kTilesOpExpect)
         {
             for (size_t i = 0; i < hg.mem.size();)
             {
@@ -552,6 +572,94 @@ fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *stat
                     : launch(&fpt::k_op_expect_tiled<float, false, 128>, 1024, sp, op);
     return wide ? launch(&fpt::k_op_expect_tiled<float, true, 64>, 512, sp, op)
                 : launch(&fpt::k_op_expect_tiled<float, false, 64>, 512, sp, op);
+}
+
+// ---------------------------------------------------------------------------
+// PauliOp::apply on coset tiles (groups of consecutive strings, one launch each)
+// ---------------------------------------------------------------------------
+
+// 0: streaming kernel, 1: tiled kernel, per FAST_PAULI_APPLY_KERNEL ("stream",
+// "tiled") or, by default, when the state is large (64 MiB and up: beyond the
+// caches that serve the streaming kernel's repeated row reads) and the groups'
+// traffic (about three state passes each) is below the strings' (one each).
+fp_status choose_op_apply_tiled(fp_hip_ops const *ops, int dtype, size_t n_states, bool *use_tiled)
+{
+    *use_tiled = false;
+    char const *v = std::getenv("FAST_PAULI_APPLY_KERNEL");
+    if (v && std::strcmp(v, "stream") == 0)
+        return FP_OK;
+    bool const forced = v && std::strcmp(v, "tiled") == 0;
+    size_t const state_bytes = (size_t(1) << ops->n_qubits) * n_states * (dtype == FP_C128 ? 16 : 8);
+    if (!forced && (ops->n_strings < 8 || state_bytes < (size_t(64) << 20)))
+        return FP_OK;
+    TileGeom const geo = tile_geom(dtype, ops->n_qubits, n_states);
+    TileTables *tt = nullptr;
+    fp_status const st = tile_tables(ops, kTilesOpApply, geo.k, geo.dh, &tt);
+    if (st != FP_OK)
+        return st;
+    *use_tiled = forced || 3.0 * double(tt->n_groups) <= 0.8 * double(ops->n_strings);
+    return FP_OK;
+}
+
+fp_status run_op_apply_tiled(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim,
+                             size_t n_states, int accumulate, hipStream_t st)
+{
+    TileGeom const geo = tile_geom(dtype, ops->n_qubits, n_states);
+    TileTables *tt = nullptr;
+    fp_status s = tile_tables(ops, kTilesOpApply, geo.k, geo.dh, &tt);
+    if (s != FP_OK)
+        return s;
+    if ((s = tile_device(ops, tt, st)) != FP_OK)
+        return s;
+    uint64_t const n_cosets = uint64_t(dim) >> geo.d;
+    uint32_t const chunks_y = uint32_t((n_states + (size_t(1) << geo.lsb) - 1) >> geo.lsb);
+    if (chunks_y > 65535u)
+        return fail(FP_ERR_INVALID_ARGUMENT, "too many states for the tiled apply kernel");
+    uint64_t const blocks_x =
+        std::max<uint64_t>(1, std::min<uint64_t>(n_cosets, (env_u32("FAST_PAULI_TILE_BLOCKS", 8192) + chunks_y - 1) /
+                                                               chunks_y));
+    uint32_t const sshift = (n_states & (n_states - 1)) == 0 ? log2u(n_states) : 64u;
+    bool const wide = (uint64_t(dim) * n_states) >> 32 != 0 || env_u32("FAST_PAULI_TILE_WIDE", 0) == 1;
+    static std::once_flag attr_once;
+    static hipError_t attr_err = hipSuccess;
+    std::call_once(attr_once, [] {
+        void const *fns[] = {reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<double, false>),
+                             reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<double, true>),
+                             reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<float, false>),
+                             reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<float, true>)};
+        for (void const *f : fns)
+        {
+            hipError_t const e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, int(fpa::kTileBytes));
+            if (e != hipSuccess && attr_err == hipSuccess)
+                attr_err = e;
+        }
+    });
+    if (attr_err != hipSuccess)
+        return fail(FP_ERR_RUNTIME, "hipFuncSetAttribute failed: %s", hipGetErrorString(attr_err));
+    dim3 const grid(uint32_t(blocks_x), chunks_y);
+    auto const *strs = static_cast<fpa::ApplyStr const *>(tt->strs);
+    for (uint32_t g = 0; g < tt->n_groups; ++g)
+    {
+        int const init_zero = (g == 0 && !accumulate) ? 1 : 0;
+        if (dtype == FP_C128)
+        {
+            typedef fpk::VecOf<double, 1>::type V;
+            auto const kfn = wide ? &fpa::k_op_apply_tiled<double, true> : &fpa::k_op_apply_tiled<double, false>;
+            hipLaunchKernelGGL(kfn, grid, dim3(fpa::kThreads), fpa::kTileBytes, st, static_cast<V const *>(states),
+                               static_cast<V *>(out), uint64_t(n_states), sshift, geo.lsb, geo.k, geo.d, n_cosets,
+                               tt->groups + g, tt->btab, strs, init_zero);
+        }
+        else
+        {
+            typedef fpk::VecOf<float, 1>::type V;
+            auto const kfn = wide ? &fpa::k_op_apply_tiled<float, true> : &fpa::k_op_apply_tiled<float, false>;
+            hipLaunchKernelGGL(kfn, grid, dim3(fpa::kThreads), fpa::kTileBytes, st, static_cast<V const *>(states),
+                               static_cast<V *>(out), uint64_t(n_states), sshift, geo.lsb, geo.k, geo.d, n_cosets,
+                               tt->groups + g, tt->btab, strs, init_zero);
+        }
+        FP_LAUNCHED();
+    }
+    return FP_OK;
 }
 
 // ---------------------------------------------------------------------------

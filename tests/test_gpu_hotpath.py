@@ -109,6 +109,51 @@ def test_op_apply_bit_exact(hip, dtype, n, N, S):
     assert np.array_equal(got, exp)
 
 
+@pytest.mark.parametrize("kernel", ["stream", "tiled", "tiled_wide"])
+@pytest.mark.parametrize("dtype", [C128, C64])
+@pytest.mark.parametrize("kind,n,N,S", [
+    ("random", 2, 5, 1), ("random", 8, 40, 3), ("random", 11, 64, 16), ("random", 13, 200, 7),
+    ("random", 15, 50, 33), ("random", 16, 300, 2), ("weight2", 14, 300, 5), ("diagonal", 12, 40, 16),
+    ("shared_x", 12, 64, 4), ("duplicates", 10, 33, 1),
+])
+def test_op_apply_kernels(hip, monkeypatch, kernel, dtype, kind, n, N, S):
+    """PauliOp::apply through the streaming kernel and the coset-tiled kernel
+    (groups of consecutive strings, one launch each): IEEE-equal to the oracle's
+    sequential loop, overwrite and accumulate."""
+    H.say(f"op_apply kernel={kernel} {kind} n={n} N={N} S={S} dtype={dtype}")
+    monkeypatch.setenv("FAST_PAULI_APPLY_KERNEL", kernel.split("_")[0])
+    if kernel == "tiled_wide":
+        monkeypatch.setenv("FAST_PAULI_TILE_WIDE", "1")
+    cdt = H.CDT[dtype]
+    codes = _structured_codes(kind, N, n, seed=7 * n + N)
+    strings = syn.codes_to_strings(codes)
+    coeffs = syn.random_coeffs(N, seed=N + 3)
+    ops = _abi.Ops(codes, coeffs)
+    psi = syn.random_states(1 << n, S, seed=n + 5 * S).astype(cdt)
+    got = H.op_apply(ops, psi, dtype)
+    exp = O.op_apply(strings, coeffs.astype(cdt), psi, np.zeros_like(psi), dtype=odt(dtype))
+    assert np.array_equal(got, exp)
+    base = syn.random_states(1 << n, S, seed=n + 11).astype(cdt)
+    got = H.op_apply(ops, psi, dtype, out0=base)
+    exp = O.op_apply(strings, coeffs.astype(cdt), psi, base.copy(), dtype=odt(dtype))
+    assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
+def test_op_apply_tiled_is_default_for_large_states(hip, monkeypatch, dtype):
+    """At 2^21 x 8 states (128 / 64 MiB) with 96 random strings the automatic
+    choice takes the tiled kernel; it equals the streaming kernel bit for bit."""
+    n, N, S = 21, 96, 8
+    H.say(f"op_apply large state auto vs stream dtype={dtype}")
+    cdt = H.CDT[dtype]
+    ops = _abi.Ops(syn.random_codes(N, n, seed=41), syn.random_coeffs(N, seed=41))
+    psi = syn.random_states(1 << n, S, seed=41).astype(cdt)
+    auto = H.op_apply(ops, psi, dtype)
+    monkeypatch.setenv("FAST_PAULI_APPLY_KERNEL", "stream")
+    ref = H.op_apply(ops, psi, dtype)
+    assert np.array_equal(auto, ref)
+
+
 def test_op_apply_known_answers(hip):
     H.say("op_apply known answers")
     a = np.arange(64).astype(np.complex128).reshape(8, 8)
