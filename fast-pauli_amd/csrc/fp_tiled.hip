@@ -410,7 +410,12 @@ fp_status tile_device(fp_hip_ops const *ops, TileTables *t, hipStream_t stream)
 {
     std::lock_guard<std::mutex> lock(ops->tiles_mu);
     if (t->on_device)
+    {
+        // the upload may have been issued on another stream: order this one after it
+        if (t->uploaded)
+            FP_HIP(hipStreamWaitEvent(stream, t->uploaded, 0));
         return FP_OK;
+    }
     size_t const total = t->h.size();
     if (t->cap < total)
     {

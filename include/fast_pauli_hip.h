@@ -127,7 +127,9 @@ fp_status fp_hip_ops_destroy(fp_hip_ops *ops);
 /* Re-target an existing set to new strings and coefficients, uploading on `stream`
  * (asynchronously when the host tables were not reused by a call still in flight);
  * allocates only when the set grows.  No reference counterpart: lets a serving loop
- * reuse operator sets without allocating.  Drops an attached coefficient matrix. */
+ * reuse operator sets without allocating.  Drops an attached coefficient matrix.
+ * Work that uses the previous strings must have completed, or be ordered before the
+ * next use of the set (the same stream): device tables are rewritten in place. */
 fp_status fp_hip_ops_assign(fp_hip_ops *ops, size_t n_qubits, size_t n_strings, uint8_t const *codes,
                             double const *coeffs, void *stream);
 /* Replace the per-string coefficients (PauliOp::scale, extend, ...). */
