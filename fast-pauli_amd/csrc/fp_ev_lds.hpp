@@ -100,15 +100,17 @@ __global__ __launch_bounds__(256) void k_transpose(V const *__restrict__ in, V *
     }
 }
 
-template <class T, int U>
-__device__ __forceinline__ void chunk_rows(typename fpk::VecOf<T, 1>::type const *col, uint32_t lane, EvChunk const &ch,
-                                           uint32_t n_blocks64, uint32_t const (&zhi)[kChunk],
-                                           uint32_t const (&lsign)[kChunk], bool const (&imag)[kChunk],
-                                           typename fpk::RowAcc<T>::type (&acc)[kChunk])
+// Rows of one chunk with exactly CNT strings.  The wave-uniform row sign bit
+// (popcount(u & zhi) parity) is applied as a scalar +-1 factor; the per-lane bit
+// is applied by the caller once per chunk.  Re w / Im w per string is a select
+// on a wave-uniform flag.
+template <class T, int U, int CNT>
+__device__ __forceinline__ void chunk_rows_n(typename fpk::VecOf<T, 1>::type const *col, uint32_t lane,
+                                             EvChunk const &ch, uint32_t n_blocks64, uint32_t const (&zhi)[kChunk],
+                                             bool const (&imag)[kChunk], typename fpk::RowAcc<T>::type (&acc)[kChunk])
 {
     typedef typename fpk::RowAcc<T>::type A;
     uint32_t const x = ch.x;
-    uint32_t const count = ch.count;
     bool const pairs = ch.mode == kPairs;
     uint32_t const hh = pairs ? ch.h - 6 : 0;
     uint32_t const lo_mask = (1u << hh) - 1;
@@ -131,21 +133,35 @@ __device__ __forceinline__ void chunk_rows(typename fpk::VecOf<T, 1>::type const
         {
             if (q0 + k < n_iter)
             {
-            A const ar = A(a[k][0]), ai = A(a[k][1]), br = A(b[k][0]), bi = A(b[k][1]);
-            A const gre = ar * br + ai * bi;
-            A const gim = ar * bi - ai * br;
+                A const ar = A(a[k][0]), ai = A(a[k][1]), br = A(b[k][0]), bi = A(b[k][1]);
+                A const gre = fma(ar, br, ai * bi);
+                A const gim = fma(ar, bi, -(ai * br));
 #pragma unroll
-            for (int s = 0; s < kChunk; ++s)
-            {
-                if (s < int(count))
+                for (int s = 0; s < CNT; ++s)
                 {
-                    uint32_t const ub = uint32_t(__popc(u[k] & zhi[s]) & 1) << 31;
-                    A const g = imag[s] ? gim : gre;
-                    acc[s] += flip_sign(g, lsign[s] ^ ub);
+                    A const sg = (__popc(u[k] & zhi[s]) & 1) ? A(-1) : A(1);
+                    acc[s] = fma(sg, imag[s] ? gim : gre, acc[s]);
                 }
             }
-            }
         }
+    }
+}
+
+template <class T, int U>
+__device__ __forceinline__ void chunk_rows(typename fpk::VecOf<T, 1>::type const *col, uint32_t lane, EvChunk const &ch,
+                                           uint32_t n_blocks64, uint32_t const (&zhi)[kChunk],
+                                           bool const (&imag)[kChunk], typename fpk::RowAcc<T>::type (&acc)[kChunk])
+{
+    switch (ch.count)
+    {
+    case 1: chunk_rows_n<T, U, 1>(col, lane, ch, n_blocks64, zhi, imag, acc); break;
+    case 2: chunk_rows_n<T, U, 2>(col, lane, ch, n_blocks64, zhi, imag, acc); break;
+    case 3: chunk_rows_n<T, U, 3>(col, lane, ch, n_blocks64, zhi, imag, acc); break;
+    case 4: chunk_rows_n<T, U, 4>(col, lane, ch, n_blocks64, zhi, imag, acc); break;
+    case 5: chunk_rows_n<T, U, 5>(col, lane, ch, n_blocks64, zhi, imag, acc); break;
+    case 6: chunk_rows_n<T, U, 6>(col, lane, ch, n_blocks64, zhi, imag, acc); break;
+    case 7: chunk_rows_n<T, U, 7>(col, lane, ch, n_blocks64, zhi, imag, acc); break;
+    default: chunk_rows_n<T, U, kChunk>(col, lane, ch, n_blocks64, zhi, imag, acc); break;
     }
 }
 
@@ -200,13 +216,13 @@ __global__ __launch_bounds__(kThreads) void k_ev_lds(typename fpk::VecOf<T, 1>::
                 imag[s] = st.imag != 0;
             }
         }
-        chunk_rows<T, 4>(col, lane, ch, n_blocks64, zhi, lsign, imag, acc);
+        chunk_rows<T, 4>(col, lane, ch, n_blocks64, zhi, imag, acc);
 #pragma unroll
         for (int s = 0; s < kChunk; ++s)
         {
             if (s < int(ch.count))
             {
-                A const sum = wave_sum(acc[s]);
+                A const sum = wave_sum(flip_sign(acc[s], lsign[s]));
                 if (lane == 0)
                 {
                     EvStr const st = strs[ch.first + s];
@@ -286,13 +302,13 @@ __global__ __launch_bounds__(kThreads) void k_ev_coset(typename fpk::VecOf<T, 1>
                 imag[s] = st.imag != 0;
             }
         }
-        chunk_rows<T, 4>(col, lane, ch, n_blocks64, zhi, lsign, imag, acc);
+        chunk_rows<T, 4>(col, lane, ch, n_blocks64, zhi, imag, acc);
 #pragma unroll
         for (int s = 0; s < kChunk; ++s)
         {
             if (s < int(ch.count))
             {
-                A const sum = wave_sum(acc[s]);
+                A const sum = wave_sum(flip_sign(acc[s], lsign[s]));
                 if (lane == 0)
                 {
                     EvStr const st = strs[ch.first + s];
