@@ -104,6 +104,21 @@ __global__ __launch_bounds__(256) void k_transpose(V const *__restrict__ in, V *
 // (popcount(u & zhi) parity) is applied as a scalar +-1 factor; the per-lane bit
 // is applied by the caller once per chunk.  Re w / Im w per string is a select
 // on a wave-uniform flag.
+template <class T> __device__ __forceinline__ T xor_sign(T v, uint32_t bit31)
+{
+    if constexpr (sizeof(T) == 4)
+        return __uint_as_float(__float_as_uint(v) ^ bit31);
+    else
+        return __longlong_as_double(__double_as_longlong(v) ^ (static_cast<long long>(bit31) << 32));
+}
+
+// Rows of one chunk with exactly CNT strings.  Iterations q = q0 | k (q0 a
+// multiple of U, k < U) visit row blocks u(q) = insert(q) = insert(q0) | insert(k)
+// (insert_zero_bit at hh is linear over GF(2)), so the wave-uniform sign bit
+// parity(u & zhi) splits into a per-chunk constant per (string, k), held as a
+// +-1 factor, and one flip bit per (string, q0).  The per-lane bit is applied by
+// the caller once per chunk.  Re w / Im w per string is a select on a
+// wave-uniform flag.
 template <class T, int U, int CNT>
 __device__ __forceinline__ void chunk_rows_n(typename fpk::VecOf<T, 1>::type const *col, uint32_t lane,
                                              EvChunk const &ch, uint32_t n_blocks64, uint32_t const (&zhi)[kChunk],
@@ -115,34 +130,53 @@ __device__ __forceinline__ void chunk_rows_n(typename fpk::VecOf<T, 1>::type con
     uint32_t const hh = pairs ? ch.h - 6 : 0;
     uint32_t const lo_mask = (1u << hh) - 1;
     uint32_t const n_iter = pairs ? (n_blocks64 >> 1) : n_blocks64;
-    for (uint32_t q0 = 0; q0 < n_iter; q0 += U)
+    auto ins = [&](uint32_t q) { return pairs ? (((q >> hh) << (hh + 1)) | (q & lo_mask)) : q; };
+    A sgk[CNT][U];
+#pragma unroll
+    for (int s = 0; s < CNT; ++s)
+#pragma unroll
+        for (int k = 0; k < U; ++k)
+            sgk[s][k] = (__popc(ins(uint32_t(k)) & zhi[s]) & 1) ? A(-1) : A(1);
+    uint32_t const n_full = n_iter & ~uint32_t(U - 1);
+    for (uint32_t q0 = 0; q0 < n_full; q0 += U)
     {
+        uint32_t const uq0 = ins(q0);
         typename fpk::VecOf<T, 1>::type a[U], b[U];
-        uint32_t u[U];
 #pragma unroll
         for (int k = 0; k < U; ++k)
         {
-            uint32_t const q = q0 + k < n_iter ? q0 + k : q0;
-            u[k] = pairs ? (((q >> hh) << (hh + 1)) | (q & lo_mask)) : q;
-            uint32_t const j = (u[k] << 6) | lane;
+            uint32_t const j = ((uq0 | ins(uint32_t(k))) << 6) | lane;
             a[k] = col[j];
             b[k] = col[j ^ x];
         }
+        uint32_t flip[CNT];
+#pragma unroll
+        for (int s = 0; s < CNT; ++s)
+            flip[s] = uint32_t(__popc(uq0 & zhi[s]) & 1) << 31;
 #pragma unroll
         for (int k = 0; k < U; ++k)
         {
-            if (q0 + k < n_iter)
-            {
-                A const ar = A(a[k][0]), ai = A(a[k][1]), br = A(b[k][0]), bi = A(b[k][1]);
-                A const gre = fma(ar, br, ai * bi);
-                A const gim = fma(ar, bi, -(ai * br));
+            A const ar = A(a[k][0]), ai = A(a[k][1]), br = A(b[k][0]), bi = A(b[k][1]);
+            A const gre = fma(ar, br, ai * bi);
+            A const gim = fma(ar, bi, -(ai * br));
 #pragma unroll
-                for (int s = 0; s < CNT; ++s)
-                {
-                    A const sg = (__popc(u[k] & zhi[s]) & 1) ? A(-1) : A(1);
-                    acc[s] = fma(sg, imag[s] ? gim : gre, acc[s]);
-                }
-            }
+            for (int s = 0; s < CNT; ++s)
+                acc[s] = fma(xor_sign(sgk[s][k], flip[s]), imag[s] ? gim : gre, acc[s]);
+        }
+    }
+    for (uint32_t q = n_full; q < n_iter; ++q) // fewer than U row blocks left
+    {
+        uint32_t const u = ins(q);
+        uint32_t const j = (u << 6) | lane;
+        typename fpk::VecOf<T, 1>::type const av = col[j], bv = col[j ^ x];
+        A const ar = A(av[0]), ai = A(av[1]), br = A(bv[0]), bi = A(bv[1]);
+        A const gre = fma(ar, br, ai * bi);
+        A const gim = fma(ar, bi, -(ai * br));
+#pragma unroll
+        for (int s = 0; s < CNT; ++s)
+        {
+            A const sg = (__popc(u & zhi[s]) & 1) ? A(-1) : A(1);
+            acc[s] = fma(sg, imag[s] ? gim : gre, acc[s]);
         }
     }
 }
