@@ -64,11 +64,34 @@ def parse():
     ap.add_argument("--pmc-pass", action="store_true",
                     help="only run the profiled kernels once each (a plain copy of the state, apply_batch, "
                          "expectation_value) for hardware-counter runs")
+    ap.add_argument("--pmc-gemm", action="store_true",
+                    help="only run the SummedPauliOp contraction (200 x 2000) x (2000 x 1000) once per MFMA mode "
+                         "(FP64, FP32, BF16x3) for hardware-counter runs")
     return ap.parse_args()
+
+
+def pmc_gemm() -> int:
+    """The n=14 SummedPauliOp contraction shape, once per contraction mode."""
+    from fast_pauli import _abi
+    from fast_pauli import synthetic as syn
+
+    _abi.init(0)
+    K, N, S = 200, 2000, 1000
+    for mode, dt in ((_abi.FP_CONTRACT_FP64, np.float64), (_abi.FP_CONTRACT_FP32, np.float32),
+                     (_abi.FP_CONTRACT_BF16X3, np.float32)):
+        A = _abi.DeviceBuffer.from_array(syn.random_real_matrix(K, N, seed=18).astype(dt))
+        B = _abi.DeviceBuffer.from_array(syn.random_real_matrix(N, S, seed=1).astype(dt))
+        C = _abi.DeviceBuffer(K * S * np.dtype(dt).itemsize)
+        _abi.call("fp_hip_gemm", mode, K, S, N, A.ptr, B.ptr, C.ptr, None)
+        _abi.synchronize()
+    log("pmc gemm pass done: FP64, FP32, BF16x3 contraction 200 x 2000 x 1000")
+    return 0
 
 
 def main() -> int:
     args = parse()
+    if args.pmc_gemm:
+        return pmc_gemm()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is "
                           "design-only (see DESIGN.md)", "n_gpus": args.gpus}), flush=True)
