@@ -50,7 +50,7 @@ template <int TKB> struct TileCfg
     static constexpr uint32_t kBytes = uint32_t(TKB) * 1024u;
     static constexpr int kThreads = TKB * 8;
 };
-constexpr int kMaxDh = 9; // string-span dimensions per tile (B table of 2^9 entries)
+constexpr int kMaxDh = 13; // string-span dimensions per tile (B table of 2^13 entries, read once per block)
 
 struct TileGroup
 {
@@ -108,6 +108,7 @@ template <class T, bool WIDE, int TKB>
 __global__ __launch_bounds__(TKB * 8) void k_op_expect_tiled(typename fpk::VecOf<T, 1>::type const *__restrict__ states,
                                                               uint64_t n_states, uint32_t sshift, uint32_t lsb,
                                                               uint32_t k, uint32_t d, uint32_t ranges,
+                                                              uint32_t n_parts, uint32_t n_chunks_y,
                                                               uint64_t n_cosets,
                                                               TileGroup const *__restrict__ groups,
                                                               uint64_t const *__restrict__ btab,
@@ -128,13 +129,22 @@ __global__ __launch_bounds__(TKB * 8) void k_op_expect_tiled(typename fpk::VecOf
     V *tile = reinterpret_cast<V *>(smem);
 
     uint32_t const tid = threadIdx.x;
-    uint32_t const g = blockIdx.x / ranges, rg = blockIdx.x % ranges;
+    // Block id -> (part, state chunk): the state chunks of one part get ids
+    // 8 apart within a run of 8 * n_chunks_y ids, i.e. the same XCD (workgroups
+    // are dealt round-robin over the 8 XCDs) at about the same time, so their
+    // slices of each row share that XCD's L2 / the Infinity Cache.
+    uint32_t const id = blockIdx.x;
+    uint32_t const part = (id / 8u / n_chunks_y) * 8u + (id % 8u);
+    uint32_t const chunk_y = (id / 8u) % n_chunks_y;
+    if (part >= n_parts)
+        return; // padding of the part count to a multiple of 8 (whole block, before any barrier)
+    uint32_t const g = part / ranges, rg = part % ranges;
     TileGroup const grp = groups[g];
     uint64_t const *bt = btab + grp.btab_off;
 
     uint32_t const sblk = 1u << lsb;
     uint32_t const t = tid & (sblk - 1u);
-    uint64_t const s0 = uint64_t(blockIdx.y) << lsb;
+    uint64_t const s0 = uint64_t(chunk_y) << lsb;
     bool const t_live = s0 + t < n_states;
     uint32_t const tile_elems = (1u << d) << lsb;
     uint32_t const lowmask = (1u << k) - 1u;
@@ -405,7 +415,7 @@ __global__ __launch_bounds__(TKB * 8) void k_op_expect_tiled(typename fpk::VecOf
             sr += red[j];
             si += red[kThreads + j];
         }
-        partial[uint64_t(blockIdx.x) * n_states + s0 + tid] = fpk::Dbl2{sr, si};
+        partial[uint64_t(part) * n_states + s0 + tid] = fpk::Dbl2{sr, si};
     }
 }
 

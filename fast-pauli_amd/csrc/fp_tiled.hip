@@ -545,10 +545,13 @@ fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *stat
     });
     if (attr_err != hipSuccess)
         return fail(FP_ERR_RUNTIME, "hipFuncSetAttribute failed: %s", hipGetErrorString(attr_err));
-    dim3 const grid(uint32_t(n_parts), chunks_y);
+    uint64_t const n_ids = (n_parts + 7) / 8 * 8 * chunks_y;
+    if (n_ids > 0x7fffffffu)
+        return fail(FP_ERR_INVALID_ARGUMENT, "too many tile blocks");
+    dim3 const grid = dim3(static_cast<uint32_t>(n_ids));
     auto launch = [&](auto kfn, int threads, auto const *typed_states, auto *typed_out) -> fp_status {
         hipLaunchKernelGGL(kfn, grid, dim3(threads), lds, st, typed_states, uint64_t(n_states), sshift, geo.lsb, geo.k,
-                           geo.d, ranges, n_cosets, tt->groups, tt->btab,
+                           geo.d, ranges, uint32_t(n_parts), chunks_y, n_cosets, tt->groups, tt->btab,
                            static_cast<fpt::TileChunk const *>(tt->chunks), static_cast<fpt::TileStr const *>(tt->strs),
                            partial);
         FP_LAUNCHED();
