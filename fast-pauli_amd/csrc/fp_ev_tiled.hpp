@@ -267,25 +267,46 @@ __global__ __launch_bounds__(TKB * 8) void k_op_expect_tiled(typename fpk::VecOf
                 auto run = [&](auto im_tag) {
                     constexpr bool IM = decltype(im_tag)::value;
                     uint32_t const zr = zt[0] >> r;
-                    uint32_t lb[U1];
+                    // Leader a = a_first + a_it and partner a ^ A = (a_first ^ A_lo) +
+                    // (a_it ^ A_hi) + 2^h, with A_lo / A_hi the bits of A in the lane /
+                    // pass part of the row index (disjoint bit ranges, so + is ^).  Both
+                    // byte addresses are a per-lane pointer plus a wave-uniform offset;
+                    // the pass part splits again into uu (per lane) and i0 (uniform).
+                    uint32_t const lr = ((1u << r) - 1u) + (((1u << r) - 1u) & hmask);
+                    uint32_t const A_lo = A & lr;
+                    uint32_t const A_hi = A ? (A & ~lr & ~(1u << h)) : 0u;
+                    uint32_t const ym0 = uint32_t(U1 - 1) << r;
+                    uint32_t const ymask = ym0 + (ym0 & hmask);
+                    uint32_t const A_hy = A_hi & ymask, A_hx = A_hi & ~ymask;
+                    uint32_t const bh = A ? (1u << h) : 0u;
+                    auto a_it = [&](uint32_t it) {
+                        uint32_t const p = it << r;
+                        return p + (p & hmask);
+                    };
+                    unsigned char const *pa0 = lds + (((a_first << lsb) | t) * kEsz);
+                    unsigned char const *pb0 =
+                        lds + ((((a_first ^ A_lo) << lsb) | t) * kEsz) + ((bh << lsb) * kEsz);
+                    unsigned char const *pa[U1];
+                    unsigned char const *pb[U1];
                     Acc pu[U1];
 #pragma unroll
                     for (int uu = 0; uu < U1; ++uu)
                     {
-                        lb[uu] = off0 ^ uniform_off(uint32_t(uu));
+                        pa[uu] = pa0 + ((a_it(uint32_t(uu)) << lsb) * kEsz);
+                        pb[uu] = pb0 + (((a_it(uint32_t(uu)) ^ A_hy) << lsb) * kEsz);
                         pu[uu] = Acc(0);
                     }
                     for (uint32_t i0 = it_begin; i0 < it_end; i0 += U1)
                     {
-                        uint32_t const ub = uniform_off(i0);
+                        uint32_t const sa = (a_it(i0) << lsb) * kEsz;
+                        uint32_t const sb = ((a_it(i0) ^ A_hx) << lsb) * kEsz;
                         Acc const sg = (__popc(i0 & zr) & 1) ? Acc(-1) : Acc(1);
                         V va[U1], vb[U1];
 #pragma unroll
                         for (int uu = 0; uu < U1; ++uu)
                         {
-                            uint32_t const addr = lb[uu] ^ ub;
-                            va[uu] = *reinterpret_cast<V const *>(lds + addr);
-                            vb[uu] = *reinterpret_cast<V const *>(lds + (addr ^ pxor));
+                            va[uu] = *reinterpret_cast<V const *>(pa[uu] + sa);
+                            vb[uu] = *reinterpret_cast<V const *>(pb[uu] + sb);
                         }
 #pragma unroll
                         for (int uu = 0; uu < U1; ++uu)
