@@ -46,7 +46,7 @@ struct ApplyStr
 // One group: launch over cosets [0, n_cosets) of group g, blocks striding by
 // gridDim.x; state chunk blockIdx.y.
 template <class T, bool WIDE>
-__global__ __launch_bounds__(kThreads, 2) void k_op_apply_tiled(typename fpk::VecOf<T, 1>::type const *__restrict__ states,
+__global__ __launch_bounds__(kThreads, 4) void k_op_apply_tiled(typename fpk::VecOf<T, 1>::type const *__restrict__ states,
                                                              typename fpk::VecOf<T, 1>::type *__restrict__ out,
                                                              uint64_t n_states, uint32_t sshift, uint32_t lsb,
                                                              uint32_t k, uint32_t d, uint64_t n_cosets,
@@ -131,6 +131,14 @@ __global__ __launch_bounds__(kThreads, 2) void k_op_apply_tiled(typename fpk::Ve
             if (!init_zero && tid + m * kThreads < tile_elems && t_live)
                 acc[m] = out[index_of(cb, m)];
         }
+        // Element m of this thread is local row a_m = a_lane + (m << rm), a_lane =
+        // tid >> lsb < 2^rm (disjoint bits).  For a string (A, Z): the partner row
+        // a_m ^ A = (a_lane ^ A_lane) + ((m << rm) ^ A_m) and the sign bit
+        // parity(a_m & Z) = parity(a_lane & Z) ^ parity((m << rm) & Z): a per-lane
+        // pointer and sign per string, plus a wave-uniform offset and bit per slot m.
+        uint32_t const rm = 9u - lsb; // log2(kThreads) - lsb
+        uint32_t const a_lane = tid >> lsb;
+        uint32_t const lane_mask = (1u << rm) - 1u;
         for (uint32_t s = 0; s < n_str; ++s)
         {
             ApplyStr const st = gs[s];
@@ -146,16 +154,20 @@ __global__ __launch_bounds__(kThreads, 2) void k_op_apply_tiled(typename fpk::Ve
                 cr = st.fr;
                 ci = st.fi;
             }
-            uint32_t const pxor = (st.A << lsb) * kEsz;
+            uint32_t const A_lane = st.A & lane_mask, A_m = st.A & ~lane_mask;
+            unsigned char const *pl = smem + ((((a_lane ^ A_lane) << lsb) | t) * kEsz);
+            // lane sign folded into the coefficient (negation is exact)
+            bool const lneg = ((uint32_t(__popc(a_lane & st.Z)) ^ cs) & 1u) != 0;
+            T const lcr = lneg ? -cr : cr;
+            T const lci = lneg ? -ci : ci;
 #pragma unroll
             for (uint32_t m = 0; m < EPT; ++m)
             {
-                uint32_t const e = tid + m * kThreads;
-                V const src = *reinterpret_cast<V const *>(smem + ((e * kEsz) ^ pxor));
-                // row sign: parity(c & z) (coset, uniform) ^ parity(a & Z) (local row a = e >> lsb)
-                bool const neg = ((uint32_t(__popc((e >> lsb) & st.Z)) ^ cs) & 1u) != 0;
-                T const a = neg ? -cr : cr;
-                T const b = neg ? -ci : ci;
+                uint32_t const mrow = m << rm;
+                V const src = *reinterpret_cast<V const *>(pl + (((mrow ^ A_m) << lsb) * kEsz));
+                uint32_t const mbit = uint32_t(__popc(mrow & st.Z) & 1u) << 31;
+                T const a = fpt::xor_sign_bit(lcr, mbit);
+                T const b = fpt::xor_sign_bit(lci, mbit);
                 T orr, oi;
                 fpk::cmul_exact<T>(a, b, src[0], src[1], orr, oi);
                 acc[m][0] = fpk::add_exact<T>(acc[m][0], orr);

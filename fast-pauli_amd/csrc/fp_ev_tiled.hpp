@@ -89,6 +89,16 @@ __device__ __forceinline__ uint32_t insert_zero_bit32(uint32_t v, uint32_t b)
     return ((v >> b) << (b + 1)) | lo;
 }
 
+// v with its sign bit flipped when bit31 = 0x80000000 (exact negation)
+__device__ __forceinline__ float xor_sign_bit(float v, uint32_t bit31)
+{
+    return __uint_as_float(__float_as_uint(v) ^ bit31);
+}
+__device__ __forceinline__ double xor_sign_bit(double v, uint32_t bit31)
+{
+    return __longlong_as_double(__double_as_longlong(v) ^ (static_cast<long long>(bit31) << 32));
+}
+
 __device__ __forceinline__ uint64_t coset_base(uint64_t q, uint32_t k, TileGroup const &g)
 {
     uint64_t v = q << k;
@@ -105,7 +115,7 @@ __device__ __forceinline__ uint64_t coset_base(uint64_t q, uint32_t k, TileGroup
 // ^ parity(it & (zt >> r)): a per-lane bit applied once per chunk and a
 // wave-uniform bit per pass.
 template <class T, bool WIDE, int TKB>
-__global__ __launch_bounds__(TKB * 8) void k_op_expect_tiled(typename fpk::VecOf<T, 1>::type const *__restrict__ states,
+__global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typename fpk::VecOf<T, 1>::type const *__restrict__ states,
                                                               uint64_t n_states, uint32_t sshift, uint32_t lsb,
                                                               uint32_t k, uint32_t d, uint32_t ranges,
                                                               uint32_t n_parts, uint32_t n_chunks_y,
