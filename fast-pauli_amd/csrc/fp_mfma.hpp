@@ -26,6 +26,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cstdlib>
+
 #include "fast_pauli_hip.h"
 
 namespace fpm
@@ -266,12 +268,17 @@ __global__ __launch_bounds__(256) void k_sum_splits(T const *__restrict__ parts,
     C[i] = s;
 }
 
-// Splits of K that bring the block count near the CU count.
+// Splits of K that bring the block count near `target_blocks`
+// (FAST_PAULI_GEMM_BLOCKS, default 512: two per CU), keeping at least 4 slabs per split.
 inline uint32_t choose_splits(size_t m, size_t n, size_t k, int bk)
 {
+    size_t target = 512;
+    if (char const *v = std::getenv("FAST_PAULI_GEMM_BLOCKS"))
+        if (long const t = std::strtol(v, nullptr, 10); t > 0)
+            target = size_t(t);
     size_t const tiles = ((m + kBM - 1) / kBM) * ((n + kBN - 1) / kBN);
     size_t splits = 1;
-    while (tiles * splits * 2 <= 256 && k / (splits * 2) >= size_t(4 * bk))
+    while (tiles * splits * 2 <= target && k / (splits * 2) >= size_t(4 * bk))
         splits *= 2;
     return uint32_t(splits);
 }
