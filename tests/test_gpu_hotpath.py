@@ -309,6 +309,38 @@ def test_tiled_tables_follow_assign_and_coeffs(hip, monkeypatch, dtype):
     assert np.array_equal(got, ref)
 
 
+def test_tiled_kernels_random_shapes(hip, monkeypatch):
+    """Seeded random shapes (n 1..15, strings 1..200, states 1..40, both dtypes,
+    random / weight-2 / diagonal / shared-x / duplicate string sets), forced
+    through the tiled expectation and apply kernels, against the oracle."""
+    rng = np.random.default_rng(2026)
+    monkeypatch.setenv("FAST_PAULI_EV_KERNEL", "tiled")
+    monkeypatch.setenv("FAST_PAULI_APPLY_KERNEL", "tiled")
+    kinds = ["random", "weight2", "diagonal", "shared_x", "duplicates"]
+    for case in range(40):
+        n = int(rng.integers(1, 16))
+        N = int(rng.integers(1, 201))
+        S = int(rng.integers(1, 41))
+        kind = kinds[int(rng.integers(0, len(kinds)))] if n >= 2 else "random"
+        dtype = C128 if rng.random() < 0.5 else C64
+        H.say(f"random tiled case {case}: {kind} n={n} N={N} S={S} dtype={dtype}")
+        cdt = H.CDT[dtype]
+        codes = _structured_codes(kind, N, n, seed=1000 + case)
+        N = codes.shape[0]
+        strings = syn.codes_to_strings(codes)
+        coeffs = syn.random_coeffs(N, seed=2000 + case)
+        ops = _abi.Ops(codes, coeffs)
+        psi = syn.random_states(1 << n, S, seed=3000 + case).astype(cdt)
+        got = H.op_expectation(ops, psi, dtype)
+        exp = O.op_expectation(strings, coeffs.astype(cdt).astype(np.complex128),
+                               psi.astype(np.complex128), np.zeros(S, np.complex128))
+        tol = 1e-12 if dtype == C128 else 1e-5
+        assert H.ev_scaled_err(got, exp, coeffs, psi) <= tol, case
+        got = H.op_apply(ops, psi, dtype)
+        exp = O.op_apply(strings, coeffs.astype(cdt), psi, np.zeros_like(psi), dtype=odt(dtype))
+        assert np.array_equal(got, exp), case
+
+
 def test_expectation_known_answers_and_invariants(hip):
     H.say("expectation known answers")
     a = np.arange(8).astype(np.complex128).reshape(8, 1)
