@@ -60,7 +60,17 @@ struct TileGroup
     uint32_t btab_off; // first entry of this group's B table (2^dh entries)
     uint64_t piv_lo;   // pivot bit positions 0..7, ascending, 8 bits each
     uint64_t piv_hi;   // pivot bit positions 8..15
+    uint32_t hoist;    // bit hoist_bit(W, U1, r): every wave of a block has at most one
+                       // unit and it takes the one-string fast path (expectation tables)
+    uint32_t pad;
 };
+
+// Bit of TileGroup::hoist for W waves per block, U1 passes per fast-path step and
+// 2^r leader rows per pass (r = 6 - log2 S_blk).
+__host__ __device__ constexpr uint32_t hoist_bit(uint32_t W, uint32_t U1, uint32_t r)
+{
+    return (W == 16 ? 16u : 0u) + (U1 == 8 ? 8u : 0u) + r;
+}
 
 struct TileChunk
 {
@@ -114,7 +124,12 @@ __device__ __forceinline__ uint64_t coset_base(uint64_t q, uint32_t k, TileGroup
 // with a_it wave-uniform, and the row sign parity(p & zt) = parity(p_first & zt)
 // ^ parity(it & (zt >> r)): a per-lane bit applied once per chunk and a
 // wave-uniform bit per pass.
-template <class T, bool WIDE, int TKB>
+//
+// Per-block set-up that does not depend on the coset is done once: a wave that
+// owns a single one-string unit keeps its addresses, signs and scale in
+// registers for all of the block's cosets, and coset bases advance by one
+// deposited addition per tile (see below) instead of a pivot-insertion loop.
+template <class T, bool WIDE, int TKB, bool HOIST>
 __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typename fpk::VecOf<T, 1>::type const *__restrict__ states,
                                                               uint64_t n_states, uint32_t sshift, uint32_t lsb,
                                                               uint32_t k, uint32_t d, uint32_t ranges,
@@ -151,6 +166,12 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
     uint32_t const g = part / ranges, rg = part % ranges;
     TileGroup const grp = groups[g];
     uint64_t const *bt = btab + grp.btab_off;
+    constexpr uint32_t W = kThreads / 64;
+    uint32_t const r = 6u - lsb;
+    // groups whose waves set their unit up once run in the HOIST instance, the
+    // others in the general one (a launch may need both; each block runs in one)
+    if ((((grp.hoist >> hoist_bit(W, U1, r)) & 1u) != 0) != HOIST)
+        return; // whole block, before any barrier
 
     uint32_t const sblk = 1u << lsb;
     uint32_t const t = tid & (sblk - 1u);
@@ -158,11 +179,20 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
     bool const t_live = s0 + t < n_states;
     uint32_t const tile_elems = (1u << d) << lsb;
     uint32_t const lowmask = (1u << k) - 1u;
+    // every element of every tile of this block is live: loads and stores need no guard
+    bool const full = tile_elems == kElems && s0 + sblk <= n_states;
     // cosets rg, rg + ranges, rg + 2 ranges, ...: at any moment the blocks of a
     // group work on consecutive cosets, whose rows are neighbours in HBM (the
     // lowest coset-index bits are the lowest non-pivot row bits)
     uint64_t const q0 = rg;
     uint64_t const q_step = ranges;
+    // Coset bases: base(q) = coset_base(q) deposits the bits of q << k into the
+    // non-pivot row bits.  Addition carries over into that deposited form with
+    // the other bits set to 1, which pass carries through:
+    //     base(q + q_step) = ((base(q) | ~cmask) + base(q_step)) & cmask,
+    // cmask = base(n_cosets - 1) (n_cosets is a power of two: every coset bit).
+    uint64_t const cmask = coset_base(n_cosets - 1u, k, grp);
+    uint64_t const cstep = coset_base(q_step, k, grp);
 
     // this thread's tile elements: element tid + m * kThreads is local row
     // a_m = p_first + m * 2^r, global row coset_base ^ roff_m.  With a
@@ -183,17 +213,25 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
     }
 
     V reg[EPT];
-    auto fetch = [&](uint64_t q) {
-        uint64_t const cb = coset_base(q, k, grp);
+    auto fetch = [&](uint64_t cb) {
         Idx const cbs = Idx(s_pow2 ? (cb << sshift) : cb);
-#pragma unroll
-        for (uint32_t m = 0; m < EPT; ++m)
+        auto index = [&](uint32_t m) {
+            return s_pow2 ? Idx(cbs ^ eoff[m]) : Idx((cbs ^ eoff[m]) * Idx(n_states) + Idx(s0 + t));
+        };
+        if (full)
         {
-            reg[m] = V(0);
-            if (tid + m * kThreads < tile_elems && t_live)
+#pragma unroll
+            for (uint32_t m = 0; m < EPT; ++m)
+                reg[m] = __builtin_nontemporal_load(&states[index(m)]);
+        }
+        else
+        {
+#pragma unroll
+            for (uint32_t m = 0; m < EPT; ++m)
             {
-                Idx const idx = s_pow2 ? Idx(cbs ^ eoff[m]) : Idx((cbs ^ eoff[m]) * Idx(n_states) + Idx(s0 + t));
-                reg[m] = __builtin_nontemporal_load(&states[idx]);
+                reg[m] = V(0);
+                if (tid + m * kThreads < tile_elems && t_live)
+                    reg[m] = __builtin_nontemporal_load(&states[index(m)]);
             }
         }
     };
@@ -202,182 +240,179 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
     // u = wave, wave + W, ...; split (a power of two) gives every wave work when a
     // group has fewer chunks than waves.  Within a wave, lane = (p_first, t):
     // 2^r = 64 / S_blk leader rows per pass.
-    constexpr uint32_t W = kThreads / 64;
     uint32_t const wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63u;
-    uint32_t const r = 6u - lsb;
     uint32_t const p_first = lane >> lsb;
-    uint32_t split = 1;
-    while (grp.n_chunks * split < W && split < 64)
-        split *= 2;
-    uint32_t const n_units = grp.n_chunks * split;
+    uint32_t split_log2 = 0;
+    while ((grp.n_chunks << split_log2) < W && split_log2 < 6)
+        ++split_log2;
+    uint32_t const split = 1u << split_log2;
+    uint32_t const n_units = grp.n_chunks << split_log2;
+
+    struct Unit
+    {
+        uint32_t A, h, count, first;
+        uint32_t it_begin, it_end; // passes of this unit
+        uint32_t hmask;            // insert_zero_bit(v, h) = v + (v & hmask) (0 when A = 0)
+        uint32_t a_first;          // this lane's first leader row
+        bool lane_live;            // dead lanes (fewer leader rows than a pass) drop their sums
+        bool fast;                 // one string, whole groups of U1 passes
+    };
+    auto make_unit = [&](uint32_t u) {
+        Unit un;
+        TileChunk const ch = chunks[grp.chunk_first + (u >> split_log2)];
+        un.A = ch.A;
+        un.h = ch.h;
+        un.count = ch.count;
+        un.first = ch.first;
+        uint32_t const n_rows = un.A ? (1u << (d - 1)) : (1u << d);
+        uint32_t const n_it = (n_rows >> r) ? (n_rows >> r) : 1u;
+        uint32_t const it_per = (n_it + split - 1) >> split_log2;
+        un.it_begin = (u & (split - 1u)) * it_per;
+        uint32_t const e = un.it_begin + it_per < n_it ? un.it_begin + it_per : n_it;
+        un.it_end = e > un.it_begin ? e : un.it_begin;
+        // Dead lanes (only when a tile has fewer leader rows than a pass) read a
+        // valid row and drop their sums; the loops stay wave-uniform, so the pass
+        // index, addresses' uniform part and signs live in scalar registers.
+        un.lane_live = p_first < n_rows;
+        uint32_t const p_lane = un.lane_live ? p_first : 0u;
+        un.hmask = un.A ? ~((1u << un.h) - 1u) : 0u;
+        un.a_first = p_lane + (p_lane & un.hmask);
+        un.fast = un.count == 1 && (un.it_begin % U1) == 0 && ((un.it_end - un.it_begin) % U1) == 0;
+        return un;
+    };
+
+    // One string, whole groups of U1 passes: pass i0 + uu (i0 a multiple of U1)
+    // has uniform address term uoff(i0) ^ uoff(uu) and row sign parity(i0 & zr) ^
+    // parity(uu & zr).  Per-uu lane bases and per-uu accumulators carry the uu
+    // parts, so each group of passes costs one scalar offset and one scalar sign.
+    struct Fast
+    {
+        uint32_t it_begin, it_end, zr, hmask, A_hx, A_hy;
+        uint32_t pa0, pb0; // LDS byte offsets of this lane's first leader / partner
+        bool im, lane_neg, lane_live;
+        uint64_t zc;
+        double sre, sim;
+    };
+    auto make_fast = [&](Unit const &un) {
+        Fast f;
+        TileStr const st = strs[un.first];
+        uint32_t const A = un.A, hmask = un.hmask;
+        f.it_begin = un.it_begin;
+        f.it_end = un.it_end;
+        f.zr = st.zt >> r;
+        f.hmask = hmask;
+        f.im = st.imag != 0;
+        f.zc = st.zc;
+        f.sre = st.sre;
+        f.sim = st.sim;
+        f.lane_live = un.lane_live;
+        f.lane_neg = (__popc(p_first & st.zt) & 1) != 0;
+        // Leader a = a_first + a_it and partner a ^ A = (a_first ^ A_lo) +
+        // (a_it ^ A_hi) + 2^h, with A_lo / A_hi the bits of A in the lane / pass
+        // part of the row index (disjoint bit ranges, so + is ^).  Both byte
+        // addresses are a per-lane offset plus a wave-uniform one; the pass part
+        // splits again into uu (per lane) and i0 (uniform).
+        uint32_t const lr = ((1u << r) - 1u) + (((1u << r) - 1u) & hmask);
+        uint32_t const A_lo = A & lr;
+        uint32_t const A_hi = A ? (A & ~lr & ~(1u << un.h)) : 0u;
+        uint32_t const ym0 = uint32_t(U1 - 1) << r;
+        uint32_t const ymask = ym0 + (ym0 & hmask);
+        f.A_hy = A_hi & ymask;
+        f.A_hx = A_hi & ~ymask;
+        uint32_t const bh = A ? (1u << un.h) : 0u;
+        f.pa0 = ((un.a_first << lsb) | t) * kEsz;
+        f.pb0 = ((((un.a_first ^ A_lo) << lsb) | t) * kEsz) + ((bh << lsb) * kEsz);
+        return f;
+    };
 
     double acc_re = 0.0, acc_im = 0.0;
-    if (q0 < n_cosets)
-        fetch(q0);
-    unsigned char const *lds = smem;
-    for (uint64_t q = q0; q < n_cosets; q += q_step)
-    {
-        __syncthreads(); // the previous tile is no longer read
+    auto run_fast = [&](Fast const &f, uint64_t q, auto im_tag) {
+        constexpr bool IM = decltype(im_tag)::value;
+        Acc pu[U1];
+        uint32_t pa[U1], pb[U1]; // byte offsets of the leader / partner of pass i0 + uu, less the i0 part
 #pragma unroll
-        for (uint32_t m = 0; m < EPT; ++m)
+        for (int uu = 0; uu < U1; ++uu)
         {
-            uint32_t const e = tid + m * kThreads;
-            if (e < tile_elems)
-                tile[e] = reg[m];
+            pu[uu] = Acc(0);
+            uint32_t const p = uint32_t(uu) << r;
+            uint32_t const ai = p + (p & f.hmask);
+            pa[uu] = f.pa0 + ((ai << lsb) * kEsz);
+            pb[uu] = f.pb0 + (((ai ^ f.A_hy) << lsb) * kEsz);
         }
-        __syncthreads();
-        if (q + q_step < n_cosets)
-            fetch(q + q_step);
-        for (uint32_t u = wave; u < n_units; u += W)
+        for (uint32_t i0 = f.it_begin; i0 < f.it_end; i0 += U1)
         {
-            TileChunk const ch = chunks[grp.chunk_first + u / split];
-            uint32_t const A = ch.A;
-            uint32_t const h = ch.h;
-            uint32_t const n_rows = A ? (1u << (d - 1)) : (1u << d);
-            uint32_t const n_it = (n_rows >> r) ? (n_rows >> r) : 1u;
-            uint32_t const it_per = (n_it + split - 1) / split;
-            uint32_t const it_begin = (u % split) * it_per;
-            uint32_t const it_end = it_begin + it_per < n_it ? it_begin + it_per : n_it;
-            // Dead lanes (only when a tile has fewer leader rows than a pass) read a
-            // valid row and drop their sums; the loops stay wave-uniform, so the pass
-            // index, addresses' uniform part and signs live in scalar registers.
-            bool const lane_live = p_first < n_rows;
-            uint32_t const p_lane = lane_live ? p_first : 0u;
-            // insert_zero_bit(v, h) = v + (v & hmask) for A != 0
-            uint32_t const hmask = A ? ~((1u << h) - 1u) : 0u;
-            uint32_t const a_first = p_lane + (p_lane & hmask);
-            uint32_t const off0 = ((a_first << lsb) | t) * kEsz;
-            uint32_t const pxor = (A << lsb) * kEsz;
-            uint32_t zt[kChunk];
-            bool imag[kChunk];
-            Acc part[kChunk];
+            uint32_t const p = i0 << r;
+            uint32_t const ai = p + (p & f.hmask);
+            uint32_t const sa = (ai << lsb) * kEsz;
+            uint32_t const sb = ((ai ^ f.A_hx) << lsb) * kEsz;
+            Acc const sg = (__popc(i0 & f.zr) & 1) ? Acc(-1) : Acc(1);
+            V va[U1], vb[U1];
 #pragma unroll
-            for (int s = 0; s < kChunk; ++s)
+            for (int uu = 0; uu < U1; ++uu)
             {
-                part[s] = Acc(0);
-                zt[s] = 0;
-                imag[s] = false;
-                if (s < int(ch.count))
-                {
-                    zt[s] = strs[ch.first + s].zt;
-                    imag[s] = strs[ch.first + s].imag != 0;
-                }
+                va[uu] = *reinterpret_cast<V const *>(smem + (pa[uu] + sa));
+                vb[uu] = *reinterpret_cast<V const *>(smem + (pb[uu] + sb));
             }
-            auto uniform_off = [&](uint32_t it) {
-                uint32_t const pit = it << r;
-                return ((pit + (pit & hmask)) << lsb) * kEsz;
-            };
-            if (ch.count == 1 && (it_begin % U1) == 0 && ((it_end - it_begin) % U1) == 0)
+#pragma unroll
+            for (int uu = 0; uu < U1; ++uu)
             {
-                // One string, whole groups of U1 passes: pass i0 + uu (i0 a multiple of
-                // U1) has uniform address term uoff(i0) ^ uoff(uu) and row sign
-                // parity(i0 & zr) ^ parity(uu & zr).  Per-uu lane bases and per-uu
-                // accumulators carry the uu parts, so each group of passes costs one
-                // scalar offset and one scalar sign.
-                auto run = [&](auto im_tag) {
-                    constexpr bool IM = decltype(im_tag)::value;
-                    uint32_t const zr = zt[0] >> r;
-                    // Leader a = a_first + a_it and partner a ^ A = (a_first ^ A_lo) +
-                    // (a_it ^ A_hi) + 2^h, with A_lo / A_hi the bits of A in the lane /
-                    // pass part of the row index (disjoint bit ranges, so + is ^).  Both
-                    // byte addresses are a per-lane pointer plus a wave-uniform offset;
-                    // the pass part splits again into uu (per lane) and i0 (uniform).
-                    uint32_t const lr = ((1u << r) - 1u) + (((1u << r) - 1u) & hmask);
-                    uint32_t const A_lo = A & lr;
-                    uint32_t const A_hi = A ? (A & ~lr & ~(1u << h)) : 0u;
-                    uint32_t const ym0 = uint32_t(U1 - 1) << r;
-                    uint32_t const ymask = ym0 + (ym0 & hmask);
-                    uint32_t const A_hy = A_hi & ymask, A_hx = A_hi & ~ymask;
-                    uint32_t const bh = A ? (1u << h) : 0u;
-                    auto a_it = [&](uint32_t it) {
-                        uint32_t const p = it << r;
-                        return p + (p & hmask);
-                    };
-                    unsigned char const *pa0 = lds + (((a_first << lsb) | t) * kEsz);
-                    unsigned char const *pb0 =
-                        lds + ((((a_first ^ A_lo) << lsb) | t) * kEsz) + ((bh << lsb) * kEsz);
-                    unsigned char const *pa[U1];
-                    unsigned char const *pb[U1];
-                    Acc pu[U1];
-#pragma unroll
-                    for (int uu = 0; uu < U1; ++uu)
-                    {
-                        pa[uu] = pa0 + ((a_it(uint32_t(uu)) << lsb) * kEsz);
-                        pb[uu] = pb0 + (((a_it(uint32_t(uu)) ^ A_hy) << lsb) * kEsz);
-                        pu[uu] = Acc(0);
-                    }
-                    for (uint32_t i0 = it_begin; i0 < it_end; i0 += U1)
-                    {
-                        uint32_t const sa = (a_it(i0) << lsb) * kEsz;
-                        uint32_t const sb = ((a_it(i0) ^ A_hx) << lsb) * kEsz;
-                        Acc const sg = (__popc(i0 & zr) & 1) ? Acc(-1) : Acc(1);
-                        V va[U1], vb[U1];
-#pragma unroll
-                        for (int uu = 0; uu < U1; ++uu)
-                        {
-                            va[uu] = *reinterpret_cast<V const *>(pa[uu] + sa);
-                            vb[uu] = *reinterpret_cast<V const *>(pb[uu] + sb);
-                        }
-#pragma unroll
-                        for (int uu = 0; uu < U1; ++uu)
-                        {
-                            Acc const ar = Acc(va[uu][0]), ai = Acc(va[uu][1]);
-                            Acc const br = Acc(vb[uu][0]), bi = Acc(vb[uu][1]);
-                            Acc const gv = IM ? fma(ar, bi, -(ai * br)) : fma(ar, br, ai * bi);
-                            pu[uu] = fma(sg, gv, pu[uu]);
-                        }
-                    }
-                    Acc p0 = Acc(0);
-#pragma unroll
-                    for (int uu = 0; uu < U1; ++uu)
-                        p0 += (__popc(uint32_t(uu) & zr) & 1) ? -pu[uu] : pu[uu];
-                    part[0] = p0;
-                };
-                if (imag[0])
-                    run(std::true_type{});
-                else
-                    run(std::false_type{});
+                Acc const ar = Acc(va[uu][0]), ai_ = Acc(va[uu][1]);
+                Acc const br = Acc(vb[uu][0]), bi = Acc(vb[uu][1]);
+                Acc const gv = IM ? fma(ar, bi, -(ai_ * br)) : fma(ar, br, ai_ * bi);
+                pu[uu] = fma(sg, gv, pu[uu]);
             }
-            else if (ch.count == 1)
+        }
+        Acc p0 = Acc(0);
+#pragma unroll
+        for (int uu = 0; uu < U1; ++uu)
+            p0 += (__popc(uint32_t(uu) & f.zr) & 1) ? -pu[uu] : pu[uu];
+        if (!f.lane_live)
+            p0 = Acc(0);
+        bool const neg = f.lane_neg != ((__popcll(q & f.zc) & 1) != 0);
+        double const v = neg ? -double(p0) : double(p0);
+        acc_re += f.sre * v;
+        acc_im += f.sim * v;
+    };
+    auto run_fast_any = [&](Fast const &f, uint64_t q) {
+        if (f.im)
+            run_fast(f, q, std::true_type{});
+        else
+            run_fast(f, q, std::false_type{});
+    };
+
+    // a unit on the general paths (ragged pass range, or several strings with one A)
+    auto run_general = [&](Unit const &un, uint64_t q) {
+        uint32_t const A = un.A;
+        uint32_t const hmask = un.hmask;
+        uint32_t const it_begin = un.it_begin, it_end = un.it_end;
+        uint32_t const off0 = ((un.a_first << lsb) | t) * kEsz;
+        uint32_t const pxor = (A << lsb) * kEsz;
+        uint32_t zt[kChunk];
+        bool imag[kChunk];
+        Acc part[kChunk];
+#pragma unroll
+        for (int s = 0; s < kChunk; ++s)
+        {
+            part[s] = Acc(0);
+            zt[s] = 0;
+            imag[s] = false;
+            if (s < int(un.count))
             {
-                // one string, ragged pass range: only the needed half of w; scalar +-1 sign
-                auto run = [&](auto im_tag) {
-                    constexpr bool IM = decltype(im_tag)::value;
-                    uint32_t const zr = zt[0] >> r;
-                    Acc p0 = Acc(0);
-                    for (uint32_t i0 = it_begin; i0 < it_end; i0 += U)
-                    {
-                        V va[U], vb[U];
-#pragma unroll
-                        for (int uu = 0; uu < U; ++uu)
-                        {
-                            uint32_t const it = i0 + uu < it_end ? i0 + uu : i0;
-                            uint32_t const addr = off0 ^ uniform_off(it);
-                            va[uu] = *reinterpret_cast<V const *>(lds + addr);
-                            vb[uu] = *reinterpret_cast<V const *>(lds + (addr ^ pxor));
-                        }
-#pragma unroll
-                        for (int uu = 0; uu < U; ++uu)
-                        {
-                            if (i0 + uu < it_end)
-                            {
-                                Acc const sg = (__popc((i0 + uu) & zr) & 1) ? Acc(-1) : Acc(1);
-                                Acc const ar = Acc(va[uu][0]), ai = Acc(va[uu][1]);
-                                Acc const br = Acc(vb[uu][0]), bi = Acc(vb[uu][1]);
-                                Acc const gv = IM ? fma(ar, bi, -(ai * br)) : fma(ar, br, ai * bi);
-                                p0 = fma(sg, gv, p0);
-                            }
-                        }
-                    }
-                    part[0] = p0;
-                };
-                if (imag[0])
-                    run(std::true_type{});
-                else
-                    run(std::false_type{});
+                zt[s] = strs[un.first + s].zt;
+                imag[s] = strs[un.first + s].imag != 0;
             }
-            else
-            {
+        }
+        auto uniform_off = [&](uint32_t it) {
+            uint32_t const pit = it << r;
+            return ((pit + (pit & hmask)) << lsb) * kEsz;
+        };
+        if (un.count == 1)
+        {
+            // one string, ragged pass range: only the needed half of w; scalar +-1 sign
+            auto run = [&](auto im_tag) {
+                constexpr bool IM = decltype(im_tag)::value;
+                uint32_t const zr = zt[0] >> r;
+                Acc p0 = Acc(0);
                 for (uint32_t i0 = it_begin; i0 < it_end; i0 += U)
                 {
                     V va[U], vb[U];
@@ -386,50 +421,148 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
                     {
                         uint32_t const it = i0 + uu < it_end ? i0 + uu : i0;
                         uint32_t const addr = off0 ^ uniform_off(it);
-                        va[uu] = *reinterpret_cast<V const *>(lds + addr);
-                        vb[uu] = *reinterpret_cast<V const *>(lds + (addr ^ pxor));
+                        va[uu] = *reinterpret_cast<V const *>(smem + addr);
+                        vb[uu] = *reinterpret_cast<V const *>(smem + (addr ^ pxor));
                     }
 #pragma unroll
                     for (int uu = 0; uu < U; ++uu)
                     {
                         if (i0 + uu < it_end)
                         {
-                            uint32_t const it = i0 + uu;
+                            Acc const sg = (__popc((i0 + uu) & zr) & 1) ? Acc(-1) : Acc(1);
                             Acc const ar = Acc(va[uu][0]), ai = Acc(va[uu][1]);
                             Acc const br = Acc(vb[uu][0]), bi = Acc(vb[uu][1]);
-                            Acc const gre = fma(ar, br, ai * bi);
-                            Acc const gim = fma(ar, bi, -(ai * br));
+                            Acc const gv = IM ? fma(ar, bi, -(ai * br)) : fma(ar, br, ai * bi);
+                            p0 = fma(sg, gv, p0);
+                        }
+                    }
+                }
+                part[0] = p0;
+            };
+            if (imag[0])
+                run(std::true_type{});
+            else
+                run(std::false_type{});
+        }
+        else
+        {
+            for (uint32_t i0 = it_begin; i0 < it_end; i0 += U)
+            {
+                V va[U], vb[U];
 #pragma unroll
-                            for (int s = 0; s < kChunk; ++s)
+                for (int uu = 0; uu < U; ++uu)
+                {
+                    uint32_t const it = i0 + uu < it_end ? i0 + uu : i0;
+                    uint32_t const addr = off0 ^ uniform_off(it);
+                    va[uu] = *reinterpret_cast<V const *>(smem + addr);
+                    vb[uu] = *reinterpret_cast<V const *>(smem + (addr ^ pxor));
+                }
+#pragma unroll
+                for (int uu = 0; uu < U; ++uu)
+                {
+                    if (i0 + uu < it_end)
+                    {
+                        uint32_t const it = i0 + uu;
+                        Acc const ar = Acc(va[uu][0]), ai = Acc(va[uu][1]);
+                        Acc const br = Acc(vb[uu][0]), bi = Acc(vb[uu][1]);
+                        Acc const gre = fma(ar, br, ai * bi);
+                        Acc const gim = fma(ar, bi, -(ai * br));
+#pragma unroll
+                        for (int s = 0; s < kChunk; ++s)
+                        {
+                            if (s < int(un.count))
                             {
-                                if (s < int(ch.count))
-                                {
-                                    Acc const sg = (__popc(it & (zt[s] >> r)) & 1) ? Acc(-1) : Acc(1);
-                                    part[s] = fma(sg, imag[s] ? gim : gre, part[s]);
-                                }
+                                Acc const sg = (__popc(it & (zt[s] >> r)) & 1) ? Acc(-1) : Acc(1);
+                                part[s] = fma(sg, imag[s] ? gim : gre, part[s]);
                             }
                         }
                     }
                 }
             }
-            if (!lane_live)
-            {
-#pragma unroll
-                for (int s = 0; s < kChunk; ++s)
-                    part[s] = Acc(0);
-            }
+        }
+        if (!un.lane_live)
+        {
 #pragma unroll
             for (int s = 0; s < kChunk; ++s)
+                part[s] = Acc(0);
+        }
+#pragma unroll
+        for (int s = 0; s < kChunk; ++s)
+        {
+            if (s < int(un.count))
             {
-                if (s < int(ch.count))
-                {
-                    TileStr const st = strs[ch.first + s];
-                    bool const neg = ((__popc(p_first & zt[s]) ^ __popcll(q & st.zc)) & 1) != 0;
-                    double const v = neg ? -double(part[s]) : double(part[s]);
-                    acc_re += st.sre * v;
-                    acc_im += st.sim * v;
-                }
+                TileStr const st = strs[un.first + s];
+                bool const neg = ((__popc(p_first & zt[s]) ^ __popcll(q & st.zc)) & 1) != 0;
+                double const v = neg ? -double(part[s]) : double(part[s]);
+                acc_re += st.sre * v;
+                acc_im += st.sim * v;
             }
+        }
+    };
+
+    auto store_tile = [&]() {
+        if (full)
+        {
+#pragma unroll
+            for (uint32_t m = 0; m < EPT; ++m)
+                tile[tid + m * kThreads] = reg[m];
+        }
+        else
+        {
+#pragma unroll
+            for (uint32_t m = 0; m < EPT; ++m)
+            {
+                uint32_t const e = tid + m * kThreads;
+                if (e < tile_elems)
+                    tile[e] = reg[m];
+            }
+        }
+    };
+    uint64_t cb = 0;
+    if (q0 < n_cosets)
+    {
+        cb = coset_base(q0, k, grp);
+        fetch(cb);
+    }
+    if constexpr (HOIST)
+    {
+        // every wave has at most one unit, on the fast path: set it up once
+        bool const idle = wave >= n_units;
+        Fast fh{};
+        if (!idle)
+            fh = make_fast(make_unit(wave));
+        for (uint64_t q = q0; q < n_cosets; q += q_step)
+        {
+            __syncthreads(); // the previous tile is no longer read
+            store_tile();
+            __syncthreads();
+            uint64_t const cbn = ((cb | ~cmask) + cstep) & cmask;
+            if (q + q_step < n_cosets)
+                fetch(cbn);
+            if (!idle)
+                run_fast_any(fh, q);
+            cb = cbn;
+        }
+    }
+    else
+    {
+        for (uint64_t q = q0; q < n_cosets; q += q_step)
+        {
+            __syncthreads(); // the previous tile is no longer read
+            store_tile();
+            __syncthreads();
+            uint64_t const cbn = ((cb | ~cmask) + cstep) & cmask;
+            if (q + q_step < n_cosets)
+                fetch(cbn);
+            for (uint32_t u = wave; u < n_units; u += W)
+            {
+                Unit const un = make_unit(u);
+                if (un.fast)
+                    run_fast_any(make_fast(un), q);
+                else
+                    run_general(un, q);
+            }
+            cb = cbn;
         }
     }
     // block sum per state, in a fixed order

@@ -66,6 +66,7 @@ struct TileTables
 {
     int kind = 0;
     uint32_t k = 0, dh = 0, n_groups = 0, max_chunks = 0;
+    uint32_t hoist_all = 0, hoist_any = 0; // AND / OR of the groups' TileGroup::hoist bits
     bool valid = false;     // built for the current strings and coefficients
     bool on_device = false; // the device copy matches the host image
     std::vector<unsigned char> h;

@@ -218,6 +218,34 @@ fp_status group_strings(fp_hip_ops const *ops, uint32_t k, uint32_t dh, bool ord
     return FP_OK;
 }
 
+// TileGroup::hoist of an expectation group of dimension d = k + dh: for each
+// block shape (W waves, U1 passes per fast-path step, 2^r leader rows per pass),
+// whether every wave owns at most one unit and that unit takes the one-string
+// fast path (the unit split of k_op_expect_tiled, restated).
+uint32_t hoist_bits(fpt::TileChunk const *ch, uint32_t n_chunks, uint32_t d)
+{
+    uint32_t bits = 0;
+    for (uint32_t W : {8u, 16u})
+        for (uint32_t U1 : {4u, 8u})
+            for (uint32_t r = 0; r <= 6; ++r)
+            {
+                uint32_t split_log2 = 0;
+                while ((n_chunks << split_log2) < W && split_log2 < 6)
+                    ++split_log2;
+                bool ok = (n_chunks << split_log2) <= W;
+                uint32_t const split = 1u << split_log2;
+                for (uint32_t c = 0; ok && c < n_chunks; ++c)
+                {
+                    uint32_t const n_rows = ch[c].A ? (1u << (d - 1)) : (1u << d);
+                    uint32_t const n_it = std::max(1u, n_rows >> r);
+                    ok = ch[c].count == 1 && n_it % (split * U1) == 0;
+                }
+                if (ok)
+                    bits |= 1u << fpt::hoist_bit(W, U1, r);
+            }
+    return bits;
+}
+
 fpt::TileGroup group_header(HostGroup const &hg, std::vector<uint64_t> &btab)
 {
     fpt::TileGroup g{};
@@ -356,11 +384,18 @@ fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, 
             }
         }
         g.n_chunks = uint32_t(n_chunks) - g.chunk_first;
+        if (kind == kTilesOpExpect)
+            g.hoist = hoist_bits(reinterpret_cast<fpt::TileChunk const *>(chunks.data()) + g.chunk_first, g.n_chunks,
+                                 k + dh);
         groups.push_back(g);
     }
-    uint32_t max_chunks = 0;
+    uint32_t max_chunks = 0, hoist_all = ~0u, hoist_any = 0;
     for (auto const &g : groups)
+    {
         max_chunks = std::max(max_chunks, g.n_chunks);
+        hoist_all &= g.hoist;
+        hoist_any |= g.hoist;
+    }
 
     // reuse an invalidated entry (its host image and device memory) when there is one
     TileTables *t = nullptr;
@@ -387,6 +422,8 @@ fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, 
     t->dh = dh;
     t->n_groups = uint32_t(groups.size());
     t->max_chunks = max_chunks;
+    t->hoist_all = groups.empty() ? 0u : hoist_all;
+    t->hoist_any = hoist_any;
     auto up = [](size_t v) { return (v + 255) / 256 * 256; };
     t->o_btab = up(groups.size() * sizeof(fpt::TileGroup));
     t->o_chunks = t->o_btab + up(btab.size() * sizeof(uint64_t));
@@ -523,25 +560,23 @@ fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *stat
     static std::once_flag attr_once;
     static hipError_t attr_err = hipSuccess;
     std::call_once(attr_once, [] {
-        struct F
-        {
-            void const *fn;
-            int bytes;
-        };
-        F const fns[] = {{reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<double, false, 64>), 64 << 10},
-                         {reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<double, true, 64>), 64 << 10},
-                         {reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<float, false, 64>), 64 << 10},
-                         {reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<float, true, 64>), 64 << 10},
-                         {reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<double, false, 128>), 128 << 10},
-                         {reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<double, true, 128>), 128 << 10},
-                         {reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<float, false, 128>), 128 << 10},
-                         {reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<float, true, 128>), 128 << 10}};
-        for (F const &f : fns)
-        {
-            hipError_t const e = hipFuncSetAttribute(f.fn, hipFuncAttributeMaxDynamicSharedMemorySize, f.bytes);
+        auto set = [](void const *fn, int bytes) {
+            hipError_t const e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
             if (e != hipSuccess && attr_err == hipSuccess)
                 attr_err = e;
-        }
+        };
+        auto set4 = [&](auto T_tag, auto tkb_tag) {
+            typedef decltype(T_tag) T;
+            constexpr int TKB = decltype(tkb_tag)::value;
+            set(reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<T, false, TKB, false>), TKB << 10);
+            set(reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<T, true, TKB, false>), TKB << 10);
+            set(reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<T, false, TKB, true>), TKB << 10);
+            set(reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<T, true, TKB, true>), TKB << 10);
+        };
+        set4(double{}, std::integral_constant<int, 64>{});
+        set4(float{}, std::integral_constant<int, 64>{});
+        set4(double{}, std::integral_constant<int, 128>{});
+        set4(float{}, std::integral_constant<int, 128>{});
     });
     if (attr_err != hipSuccess)
         return fail(FP_ERR_RUNTIME, "hipFuncSetAttribute failed: %s", hipGetErrorString(attr_err));
@@ -549,37 +584,46 @@ fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *stat
     if (n_ids > 0x7fffffffu)
         return fail(FP_ERR_INVALID_ARGUMENT, "too many tile blocks");
     dim3 const grid = dim3(static_cast<uint32_t>(n_ids));
-    auto launch = [&](auto kfn, int threads, auto const *typed_states, auto *typed_out) -> fp_status {
-        hipLaunchKernelGGL(kfn, grid, dim3(threads), lds, st, typed_states, uint64_t(n_states), sshift, geo.lsb, geo.k,
-                           geo.d, ranges, uint32_t(n_parts), chunks_y, n_cosets, tt->groups, tt->btab,
-                           static_cast<fpt::TileChunk const *>(tt->chunks), static_cast<fpt::TileStr const *>(tt->strs),
-                           partial);
-        FP_LAUNCHED();
-        typedef std::remove_pointer_t<decltype(typed_out)> OutT;
-        hipLaunchKernelGGL(fpk::k_reduce_partials<OutT>, dim3(uint32_t(n_states)), dim3(fpk::kBlock), 0, st, partial,
-                           n_parts, uint64_t(n_states), typed_out, accumulate, 1.0, 0.0);
+    // Groups whose waves each own at most one fast-path unit run in the HOIST
+    // instance, the others in the general one; blocks of the other kind exit at
+    // once, so a launch with both kinds runs both instances over the same grid.
+    uint32_t const hb = fpt::hoist_bit(geo.tile_kb == 128 ? 16u : 8u, dtype == FP_C128 ? 4u : 8u, 6u - geo.lsb);
+    bool const run_hoist = ((tt->hoist_any >> hb) & 1u) != 0;
+    bool const run_general = ((tt->hoist_all >> hb) & 1u) == 0;
+    auto launch = [&](auto T_tag, auto wide_tag, auto tkb_tag, void *typed_out_v) -> fp_status {
+        typedef decltype(T_tag) T;
+        typedef typename fpk::VecOf<T, 1>::type V;
+        constexpr bool WIDE = decltype(wide_tag)::value;
+        constexpr int TKB = decltype(tkb_tag)::value;
+        auto const *typed_states = static_cast<V const *>(states);
+        for (int pass = 0; pass < 2; ++pass)
+        {
+            if (pass == 0 ? !run_hoist : !run_general)
+                continue;
+            auto const kfn = pass == 0 ? &fpt::k_op_expect_tiled<T, WIDE, TKB, true>
+                                       : &fpt::k_op_expect_tiled<T, WIDE, TKB, false>;
+            hipLaunchKernelGGL(kfn, grid, dim3(TKB * 8), lds, st, typed_states, uint64_t(n_states), sshift, geo.lsb,
+                               geo.k, geo.d, ranges, uint32_t(n_parts), chunks_y, n_cosets, tt->groups, tt->btab,
+                               static_cast<fpt::TileChunk const *>(tt->chunks),
+                               static_cast<fpt::TileStr const *>(tt->strs), partial);
+            FP_LAUNCHED();
+        }
+        hipLaunchKernelGGL(fpk::k_reduce_partials<T>, dim3(uint32_t(n_states)), dim3(fpk::kBlock), 0, st, partial,
+                           n_parts, uint64_t(n_states), static_cast<T *>(typed_out_v), accumulate, 1.0, 0.0);
         FP_LAUNCHED();
         return FP_OK;
     };
+    typedef std::integral_constant<int, 64> K64;
+    typedef std::integral_constant<int, 128> K128;
     if (dtype == FP_C128)
     {
-        typedef fpk::VecOf<double, 1>::type V;
-        auto const *sp = static_cast<V const *>(states);
-        auto *op = static_cast<double *>(out);
         if (geo.tile_kb == 128)
-            return wide ? launch(&fpt::k_op_expect_tiled<double, true, 128>, 1024, sp, op)
-                        : launch(&fpt::k_op_expect_tiled<double, false, 128>, 1024, sp, op);
-        return wide ? launch(&fpt::k_op_expect_tiled<double, true, 64>, 512, sp, op)
-                    : launch(&fpt::k_op_expect_tiled<double, false, 64>, 512, sp, op);
+            return wide ? launch(double{}, std::true_type{}, K128{}, out) : launch(double{}, std::false_type{}, K128{}, out);
+        return wide ? launch(double{}, std::true_type{}, K64{}, out) : launch(double{}, std::false_type{}, K64{}, out);
     }
-    typedef fpk::VecOf<float, 1>::type V;
-    auto const *sp = static_cast<V const *>(states);
-    auto *op = static_cast<float *>(out);
     if (geo.tile_kb == 128)
-        return wide ? launch(&fpt::k_op_expect_tiled<float, true, 128>, 1024, sp, op)
-                    : launch(&fpt::k_op_expect_tiled<float, false, 128>, 1024, sp, op);
-    return wide ? launch(&fpt::k_op_expect_tiled<float, true, 64>, 512, sp, op)
-                : launch(&fpt::k_op_expect_tiled<float, false, 64>, 512, sp, op);
+        return wide ? launch(float{}, std::true_type{}, K128{}, out) : launch(float{}, std::false_type{}, K128{}, out);
+    return wide ? launch(float{}, std::true_type{}, K64{}, out) : launch(float{}, std::false_type{}, K64{}, out);
 }
 
 // ---------------------------------------------------------------------------
