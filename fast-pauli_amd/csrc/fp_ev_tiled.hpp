@@ -168,9 +168,14 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
     uint64_t const *bt = btab + grp.btab_off;
     constexpr uint32_t W = kThreads / 64;
     uint32_t const r = 6u - lsb;
-    // groups whose waves set their unit up once run in the HOIST instance, the
-    // others in the general one (a launch may need both; each block runs in one)
-    if ((((grp.hoist >> hoist_bit(W, U1, r)) & 1u) != 0) != HOIST)
+    // Groups whose waves set their unit up once run in the HOIST instance, the
+    // others in the general one (a launch may need both; each block runs in one).
+    // HOIST runs 4 passes per step, complex64 in the paired lane form (2^(r+1)
+    // leader rows per pass), which needs S_blk >= 2.
+    constexpr bool PAIRED = HOIST && sizeof(V) == 8;
+    bool const g_hoist = (sizeof(V) != 8 || lsb >= 1) &&
+                         ((grp.hoist >> hoist_bit(W, 4u, sizeof(V) == 8 ? r + 1u : r)) & 1u) != 0;
+    if (g_hoist != HOIST)
         return; // whole block, before any barrier
 
     uint32_t const sblk = 1u << lsb;
@@ -237,16 +242,29 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
     };
 
     // Waves own work units: unit u = (chunk u / split, row part u % split),
-    // u = wave, wave + W, ...; split (a power of two) gives every wave work when a
-    // group has fewer chunks than waves.  Within a wave, lane = (p_first, t):
-    // 2^r = 64 / S_blk leader rows per pass.
+    // u = wave, wave + W, ...; split is the largest power of two with
+    // n_chunks * split <= W (1 when a group has W chunks or more), so that a group
+    // of at most W chunks gives each wave at most one unit.  Within a wave, lane =
+    // (p_first, t): 2^r = 64 / S_blk leader rows per pass.
     uint32_t const wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63u;
     uint32_t const p_first = lane >> lsb;
     uint32_t split_log2 = 0;
-    while ((grp.n_chunks << split_log2) < W && split_log2 < 6)
+    while ((grp.n_chunks << (split_log2 + 1)) <= W && split_log2 < 6)
         ++split_log2;
     uint32_t const split = 1u << split_log2;
     uint32_t const n_units = grp.n_chunks << split_log2;
+
+    // Lane geometry of the fast path: lane = (pf, tx) with 2^rr leader rows per
+    // pass; the lane reads LDS at byte ((a << lsbx) | tx) * ESZ for row a.  The
+    // plain form has one state per lane (lsbx = lsb, tx = t, rr = r, ESZ = the
+    // element size).  The paired form (complex64, S_blk >= 2) gives a lane the two
+    // states 2 tx, 2 tx + 1 with one 16-byte read (lsbx = lsb - 1, rr = r + 1):
+    // half the LDS instructions per (pair, state).
+    struct Lanes
+    {
+        uint32_t lsbx, tx, rr, pf;
+    };
+    Lanes const plain{lsb, t, r, p_first};
 
     struct Unit
     {
@@ -255,9 +273,9 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
         uint32_t hmask;            // insert_zero_bit(v, h) = v + (v & hmask) (0 when A = 0)
         uint32_t a_first;          // this lane's first leader row
         bool lane_live;            // dead lanes (fewer leader rows than a pass) drop their sums
-        bool fast;                 // one string, whole groups of U1 passes
+        bool fast;                 // one string, whole groups of UU passes
     };
-    auto make_unit = [&](uint32_t u) {
+    auto make_unit = [&](uint32_t u, Lanes const &lg, uint32_t UU) {
         Unit un;
         TileChunk const ch = chunks[grp.chunk_first + (u >> split_log2)];
         un.A = ch.A;
@@ -265,7 +283,7 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
         un.count = ch.count;
         un.first = ch.first;
         uint32_t const n_rows = un.A ? (1u << (d - 1)) : (1u << d);
-        uint32_t const n_it = (n_rows >> r) ? (n_rows >> r) : 1u;
+        uint32_t const n_it = (n_rows >> lg.rr) ? (n_rows >> lg.rr) : 1u;
         uint32_t const it_per = (n_it + split - 1) >> split_log2;
         un.it_begin = (u & (split - 1u)) * it_per;
         uint32_t const e = un.it_begin + it_per < n_it ? un.it_begin + it_per : n_it;
@@ -273,15 +291,15 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
         // Dead lanes (only when a tile has fewer leader rows than a pass) read a
         // valid row and drop their sums; the loops stay wave-uniform, so the pass
         // index, addresses' uniform part and signs live in scalar registers.
-        un.lane_live = p_first < n_rows;
-        uint32_t const p_lane = un.lane_live ? p_first : 0u;
+        un.lane_live = lg.pf < n_rows;
+        uint32_t const p_lane = un.lane_live ? lg.pf : 0u;
         un.hmask = un.A ? ~((1u << un.h) - 1u) : 0u;
         un.a_first = p_lane + (p_lane & un.hmask);
-        un.fast = un.count == 1 && (un.it_begin % U1) == 0 && ((un.it_end - un.it_begin) % U1) == 0;
+        un.fast = un.count == 1 && (un.it_begin % UU) == 0 && ((un.it_end - un.it_begin) % UU) == 0;
         return un;
     };
 
-    // One string, whole groups of U1 passes: pass i0 + uu (i0 a multiple of U1)
+    // One string, whole groups of UU passes: pass i0 + uu (i0 a multiple of UU)
     // has uniform address term uoff(i0) ^ uoff(uu) and row sign parity(i0 & zr) ^
     // parity(uu & zr).  Per-uu lane bases and per-uu accumulators carry the uu
     // parts, so each group of passes costs one scalar offset and one scalar sign.
@@ -293,91 +311,117 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
         uint64_t zc;
         double sre, sim;
     };
-    auto make_fast = [&](Unit const &un) {
+    auto make_fast = [&](Unit const &un, Lanes const &lg, uint32_t esz, uint32_t UU) {
         Fast f;
         TileStr const st = strs[un.first];
-        uint32_t const A = un.A, hmask = un.hmask;
+        uint32_t const A = un.A, hmask = un.hmask, rr = lg.rr;
         f.it_begin = un.it_begin;
         f.it_end = un.it_end;
-        f.zr = st.zt >> r;
+        f.zr = st.zt >> rr;
         f.hmask = hmask;
         f.im = st.imag != 0;
         f.zc = st.zc;
         f.sre = st.sre;
         f.sim = st.sim;
         f.lane_live = un.lane_live;
-        f.lane_neg = (__popc(p_first & st.zt) & 1) != 0;
+        f.lane_neg = (__popc(lg.pf & st.zt) & 1) != 0;
         // Leader a = a_first + a_it and partner a ^ A = (a_first ^ A_lo) +
         // (a_it ^ A_hi) + 2^h, with A_lo / A_hi the bits of A in the lane / pass
         // part of the row index (disjoint bit ranges, so + is ^).  Both byte
         // addresses are a per-lane offset plus a wave-uniform one; the pass part
         // splits again into uu (per lane) and i0 (uniform).
-        uint32_t const lr = ((1u << r) - 1u) + (((1u << r) - 1u) & hmask);
+        uint32_t const lr = ((1u << rr) - 1u) + (((1u << rr) - 1u) & hmask);
         uint32_t const A_lo = A & lr;
         uint32_t const A_hi = A ? (A & ~lr & ~(1u << un.h)) : 0u;
-        uint32_t const ym0 = uint32_t(U1 - 1) << r;
+        uint32_t const ym0 = (UU - 1u) << rr;
         uint32_t const ymask = ym0 + (ym0 & hmask);
         f.A_hy = A_hi & ymask;
         f.A_hx = A_hi & ~ymask;
         uint32_t const bh = A ? (1u << un.h) : 0u;
-        f.pa0 = ((un.a_first << lsb) | t) * kEsz;
-        f.pb0 = ((((un.a_first ^ A_lo) << lsb) | t) * kEsz) + ((bh << lsb) * kEsz);
+        f.pa0 = ((un.a_first << lg.lsbx) | lg.tx) * esz;
+        f.pb0 = ((((un.a_first ^ A_lo) << lg.lsbx) | lg.tx) * esz) + ((bh << lg.lsbx) * esz);
         return f;
     };
 
-    double acc_re = 0.0, acc_im = 0.0;
-    auto run_fast = [&](Fast const &f, uint64_t q, auto im_tag) {
+    // acc_re/acc_im: this thread's state (plain form) or the even state of its
+    // pair (paired form); acc_re1/acc_im1: the odd state of the pair
+    double acc_re = 0.0, acc_im = 0.0, acc_re1 = 0.0, acc_im1 = 0.0;
+    auto run_fast = [&](Fast const &f, uint64_t q, Lanes const &lg, auto im_tag, auto paired_tag) {
         constexpr bool IM = decltype(im_tag)::value;
-        Acc pu[U1];
-        uint32_t pa[U1], pb[U1]; // byte offsets of the leader / partner of pass i0 + uu, less the i0 part
+        constexpr bool PAIRED = decltype(paired_tag)::value;
+        typedef typename std::conditional<PAIRED, typename fpk::VecOf<float, 2>::type, V>::type LV;
+        constexpr int UU = PAIRED ? 4 : U1;
+        constexpr uint32_t ESZ = sizeof(LV);
+        constexpr int NS = PAIRED ? 2 : 1; // states per lane
+        uint32_t const rr = lg.rr, lsbx = lg.lsbx;
+        Acc pu[NS][UU];
+        uint32_t pa[UU], pb[UU]; // byte offsets of the leader / partner of pass i0 + uu, less the i0 part
 #pragma unroll
-        for (int uu = 0; uu < U1; ++uu)
+        for (int uu = 0; uu < UU; ++uu)
         {
-            pu[uu] = Acc(0);
-            uint32_t const p = uint32_t(uu) << r;
+#pragma unroll
+            for (int j = 0; j < NS; ++j)
+                pu[j][uu] = Acc(0);
+            uint32_t const p = uint32_t(uu) << rr;
             uint32_t const ai = p + (p & f.hmask);
-            pa[uu] = f.pa0 + ((ai << lsb) * kEsz);
-            pb[uu] = f.pb0 + (((ai ^ f.A_hy) << lsb) * kEsz);
+            pa[uu] = f.pa0 + ((ai << lsbx) * ESZ);
+            pb[uu] = f.pb0 + (((ai ^ f.A_hy) << lsbx) * ESZ);
         }
-        for (uint32_t i0 = f.it_begin; i0 < f.it_end; i0 += U1)
+        for (uint32_t i0 = f.it_begin; i0 < f.it_end; i0 += UU)
         {
-            uint32_t const p = i0 << r;
+            uint32_t const p = i0 << rr;
             uint32_t const ai = p + (p & f.hmask);
-            uint32_t const sa = (ai << lsb) * kEsz;
-            uint32_t const sb = ((ai ^ f.A_hx) << lsb) * kEsz;
+            uint32_t const sa = (ai << lsbx) * ESZ;
+            uint32_t const sb = ((ai ^ f.A_hx) << lsbx) * ESZ;
             Acc const sg = (__popc(i0 & f.zr) & 1) ? Acc(-1) : Acc(1);
-            V va[U1], vb[U1];
+            LV va[UU], vb[UU];
 #pragma unroll
-            for (int uu = 0; uu < U1; ++uu)
+            for (int uu = 0; uu < UU; ++uu)
             {
-                va[uu] = *reinterpret_cast<V const *>(smem + (pa[uu] + sa));
-                vb[uu] = *reinterpret_cast<V const *>(smem + (pb[uu] + sb));
+                va[uu] = *reinterpret_cast<LV const *>(smem + (pa[uu] + sa));
+                vb[uu] = *reinterpret_cast<LV const *>(smem + (pb[uu] + sb));
             }
 #pragma unroll
-            for (int uu = 0; uu < U1; ++uu)
+            for (int uu = 0; uu < UU; ++uu)
             {
-                Acc const ar = Acc(va[uu][0]), ai_ = Acc(va[uu][1]);
-                Acc const br = Acc(vb[uu][0]), bi = Acc(vb[uu][1]);
-                Acc const gv = IM ? fma(ar, bi, -(ai_ * br)) : fma(ar, br, ai_ * bi);
-                pu[uu] = fma(sg, gv, pu[uu]);
+#pragma unroll
+                for (int j = 0; j < NS; ++j)
+                {
+                    Acc const ar = Acc(va[uu][2 * j]), ai_ = Acc(va[uu][2 * j + 1]);
+                    Acc const br = Acc(vb[uu][2 * j]), bi = Acc(vb[uu][2 * j + 1]);
+                    Acc const gv = IM ? fma(ar, bi, -(ai_ * br)) : fma(ar, br, ai_ * bi);
+                    pu[j][uu] = fma(sg, gv, pu[j][uu]);
+                }
             }
         }
-        Acc p0 = Acc(0);
-#pragma unroll
-        for (int uu = 0; uu < U1; ++uu)
-            p0 += (__popc(uint32_t(uu) & f.zr) & 1) ? -pu[uu] : pu[uu];
-        if (!f.lane_live)
-            p0 = Acc(0);
         bool const neg = f.lane_neg != ((__popcll(q & f.zc) & 1) != 0);
-        double const v = neg ? -double(p0) : double(p0);
-        acc_re += f.sre * v;
-        acc_im += f.sim * v;
+#pragma unroll
+        for (int j = 0; j < NS; ++j)
+        {
+            Acc p0 = Acc(0);
+#pragma unroll
+            for (int uu = 0; uu < UU; ++uu)
+                p0 += (__popc(uint32_t(uu) & f.zr) & 1) ? -pu[j][uu] : pu[j][uu];
+            if (!f.lane_live)
+                p0 = Acc(0);
+            double const v = neg ? -double(p0) : double(p0);
+            if (j == 0)
+            {
+                acc_re += f.sre * v;
+                acc_im += f.sim * v;
+            }
+            else
+            {
+                acc_re1 += f.sre * v;
+                acc_im1 += f.sim * v;
+            }
+        }
     };
-    auto run_fast_any = [&](Fast const &f, uint64_t q) {
+    auto run_fast_any = [&](Fast const &f, uint64_t q, Lanes const &lg, auto paired_tag) {
         if (f.im)
-            run_fast(f, q, std::true_type{});
+            run_fast(f, q, lg, std::true_type{}, paired_tag);
         else
-            run_fast(f, q, std::false_type{});
+            run_fast(f, q, lg, std::false_type{}, paired_tag);
     };
 
     // a unit on the general paths (ragged pass range, or several strings with one A)
@@ -528,9 +572,11 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
     {
         // every wave has at most one unit, on the fast path: set it up once
         bool const idle = wave >= n_units;
+        uint32_t const sb2 = PAIRED ? (sblk >> 1) : sblk;
+        Lanes const lg = PAIRED ? Lanes{lsb - 1u, lane & (sb2 - 1u), r + 1u, lane >> (lsb - 1u)} : plain;
         Fast fh{};
         if (!idle)
-            fh = make_fast(make_unit(wave));
+            fh = make_fast(make_unit(wave, lg, 4u), lg, PAIRED ? 16u : kEsz, 4u);
         for (uint64_t q = q0; q < n_cosets; q += q_step)
         {
             __syncthreads(); // the previous tile is no longer read
@@ -540,7 +586,7 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
             if (q + q_step < n_cosets)
                 fetch(cbn);
             if (!idle)
-                run_fast_any(fh, q);
+                run_fast_any(fh, q, lg, std::integral_constant<bool, PAIRED>{});
             cb = cbn;
         }
     }
@@ -556,9 +602,9 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
                 fetch(cbn);
             for (uint32_t u = wave; u < n_units; u += W)
             {
-                Unit const un = make_unit(u);
+                Unit const un = make_unit(u, plain, U1);
                 if (un.fast)
-                    run_fast_any(make_fast(un), q);
+                    run_fast_any(make_fast(un, plain, kEsz, U1), q, plain, std::false_type{});
                 else
                     run_general(un, q);
             }
@@ -570,14 +616,24 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
     double *red = reinterpret_cast<double *>(smem);
     red[tid] = acc_re;
     red[kThreads + tid] = acc_im;
+    if constexpr (PAIRED)
+    {
+        red[2 * kThreads + tid] = acc_re1;
+        red[3 * kThreads + tid] = acc_im1;
+    }
     __syncthreads();
     if (tid < sblk && s0 + tid < n_states)
     {
+        // plain: threads tid + j sblk hold state tid; paired: threads t2 + j (sblk / 2)
+        // hold state 2 t2 (first pair) and 2 t2 + 1 (second pair)
+        uint32_t const stride = PAIRED ? (sblk >> 1) : sblk;
+        uint32_t const first = PAIRED ? (tid >> 1) : tid;
+        uint32_t const base = PAIRED ? (tid & 1u) * 2u * uint32_t(kThreads) : 0u;
         double sr = 0.0, si = 0.0;
-        for (uint32_t j = tid; j < uint32_t(kThreads); j += sblk)
+        for (uint32_t j = first; j < uint32_t(kThreads); j += stride)
         {
-            sr += red[j];
-            si += red[kThreads + j];
+            sr += red[base + j];
+            si += red[base + kThreads + j];
         }
         partial[uint64_t(part) * n_states + s0 + tid] = fpk::Dbl2{sr, si};
     }

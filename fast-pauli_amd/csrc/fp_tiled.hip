@@ -230,7 +230,7 @@ uint32_t hoist_bits(fpt::TileChunk const *ch, uint32_t n_chunks, uint32_t d)
             for (uint32_t r = 0; r <= 6; ++r)
             {
                 uint32_t split_log2 = 0;
-                while ((n_chunks << split_log2) < W && split_log2 < 6)
+                while ((n_chunks << (split_log2 + 1)) <= W && split_log2 < 6)
                     ++split_log2;
                 bool ok = (n_chunks << split_log2) <= W;
                 uint32_t const split = 1u << split_log2;
@@ -524,6 +524,16 @@ fp_status choose_op_expect_tiled(fp_hip_ops const *ops, int dtype, size_t n_stat
     return FP_OK;
 }
 
+// TileGroup::hoist bit that the HOIST instance of k_op_expect_tiled checks: it
+// runs 4 passes per step with 2^r leader rows per pass, r = 6 - lsb, or r + 1
+// for complex64 (two states per lane, which needs lsb >= 1).
+uint32_t ev_hoist_bit(int dtype, TileGeom const &geo, bool *can_hoist)
+{
+    bool const c64 = dtype != FP_C128;
+    *can_hoist = !c64 || geo.lsb >= 1;
+    return fpt::hoist_bit(geo.tile_kb == 128 ? 16u : 8u, 4u, (c64 ? 7u : 6u) - geo.lsb);
+}
+
 fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim,
                               size_t n_states, int accumulate, hipStream_t st)
 {
@@ -587,9 +597,10 @@ fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *stat
     // Groups whose waves each own at most one fast-path unit run in the HOIST
     // instance, the others in the general one; blocks of the other kind exit at
     // once, so a launch with both kinds runs both instances over the same grid.
-    uint32_t const hb = fpt::hoist_bit(geo.tile_kb == 128 ? 16u : 8u, dtype == FP_C128 ? 4u : 8u, 6u - geo.lsb);
-    bool const run_hoist = ((tt->hoist_any >> hb) & 1u) != 0;
-    bool const run_general = ((tt->hoist_all >> hb) & 1u) == 0;
+    bool can_hoist = false;
+    uint32_t const hb = ev_hoist_bit(dtype, geo, &can_hoist);
+    bool const run_hoist = can_hoist && ((tt->hoist_any >> hb) & 1u) != 0;
+    bool const run_general = !can_hoist || ((tt->hoist_all >> hb) & 1u) == 0;
     auto launch = [&](auto T_tag, auto wide_tag, auto tkb_tag, void *typed_out_v) -> fp_status {
         typedef decltype(T_tag) T;
         typedef typename fpk::VecOf<T, 1>::type V;
@@ -851,5 +862,10 @@ extern "C" fp_status fp_hip_op_expectation_plan(fp_hip_ops const *ops, int dtype
     plan->tile_rows_log2 = geo.d;
     plan->states_per_block = 1u << geo.lsb;
     plan->hbm_bytes = state_bytes * tt->n_groups;
+    bool can_hoist = false;
+    uint32_t const hb = fprt::ev_hoist_bit(dtype, geo, &can_hoist);
+    auto const *groups = reinterpret_cast<fpt::TileGroup const *>(tt->h.data());
+    for (uint32_t g = 0; can_hoist && g < tt->n_groups; ++g)
+        plan->hoisted_groups += (groups[g].hoist >> hb) & 1u;
     return FP_OK;
 }

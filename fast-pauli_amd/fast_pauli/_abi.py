@@ -100,10 +100,11 @@ FP_EV_STREAM, FP_EV_TILED = 0, 1
 class EvPlan(ctypes.Structure):
     """fp_hip_ev_plan"""
     _fields_ = [("kernel", ctypes.c_int), ("n_groups", ctypes.c_uint32), ("tile_rows_log2", ctypes.c_uint32),
-                ("states_per_block", ctypes.c_uint32), ("hbm_bytes", ctypes.c_uint64)]
+                ("states_per_block", ctypes.c_uint32), ("hbm_bytes", ctypes.c_uint64),
+                ("hoisted_groups", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
     def as_dict(self) -> dict:
-        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+        return {f: int(getattr(self, f)) for f, _ in self._fields_ if f != "reserved"}
 
 
 def op_expectation_plan(ops, dtype: int, dim: int, n_states: int) -> dict:

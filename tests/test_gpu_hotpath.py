@@ -271,8 +271,10 @@ def test_op_expectation_plan(hip, monkeypatch):
     assert p["tile_rows_log2"] == 8 and p["states_per_block"] == 16
     assert N // 8 <= p["n_groups"] <= N // 4
     assert p["hbm_bytes"] == p["n_groups"] * (1 << n) * S * 16
+    assert N // 16 <= p["hoisted_groups"] <= p["n_groups"]
     p64 = _abi.op_expectation_plan(ops, C64, 1 << n, S)
     assert p64["tile_rows_log2"] == 9 and p64["n_groups"] <= p["n_groups"]
+    assert p64["hoisted_groups"] <= p64["n_groups"]
     single = _abi.Ops(syn.random_codes(1, n, seed=5))
     assert _abi.op_expectation_plan(single, C128, 1 << n, S)["kernel"] == _abi.FP_EV_STREAM
     monkeypatch.setenv("FAST_PAULI_EV_KERNEL", "stream")
@@ -280,6 +282,46 @@ def test_op_expectation_plan(hip, monkeypatch):
     assert q["kernel"] == _abi.FP_EV_STREAM and q["hbm_bytes"] == N * (1 << n) * S * 16
     with pytest.raises(ValueError):
         _abi.op_expectation_plan(ops, C128, 1 << (n - 1), S)
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
+@pytest.mark.parametrize("S", [1, 2, 16, 24])
+def test_tiled_expectation_both_instances(hip, monkeypatch, dtype, S):
+    """Groups whose waves each own one single-string unit run in the set-up-once
+    instance of the tiled kernel (two states per lane for complex64, which needs
+    2 states or more per block), the others in the per-tile instance; one launch
+    holding both kinds of group equals the oracle, and so do launches of each
+    kind alone."""
+    H.say(f"tiled expectation instances dtype={dtype} S={S}")
+    monkeypatch.setenv("FAST_PAULI_EV_KERNEL", "tiled")
+    n = 16
+    cdt = H.CDT[dtype]
+    psi = syn.random_states(1 << n, S, seed=41).astype(cdt)
+    mixed = np.concatenate([syn.random_codes(64, n, seed=41), _structured_codes("shared_x", 12, n, seed=42)])
+    cases = [("random", syn.random_codes(64, n, seed=43)), ("shared_x", _structured_codes("shared_x", 40, n, seed=44)),
+             ("mixed", mixed)]
+    for name, codes in cases:
+        coeffs = syn.random_coeffs(codes.shape[0], seed=45)
+        ops = _abi.Ops(codes, coeffs)
+        p = _abi.op_expectation_plan(ops, dtype, 1 << n, S)
+        assert p["kernel"] == _abi.FP_EV_TILED
+        h = p["hoisted_groups"]
+        # a group is hoisted when it has 1, 2, 4 or 8 chunks of one string each (a
+        # random group may take more than 8 strings when x masks fall in its span)
+        if dtype == C64 and S == 1:
+            assert h == 0, name  # one state per block: no two-state lanes
+        elif name == "random":
+            assert 0 < h <= p["n_groups"], (name, p)
+        elif name == "shared_x":
+            assert h == 0, (name, p)  # strings sharing an x mask share chunks
+        else:
+            assert 0 < h < p["n_groups"], (name, p)
+        got = H.op_expectation(ops, psi, dtype)
+        exp = O.op_expectation(syn.codes_to_strings(codes), coeffs.astype(cdt).astype(np.complex128),
+                               psi.astype(np.complex128), np.zeros(S, np.complex128))
+        tol = 1e-12 if dtype == C128 else 1e-5
+        assert H.ev_scaled_err(got, exp, coeffs, psi) <= tol, name
+        assert np.array_equal(H.op_expectation(ops, psi, dtype), got), name
 
 
 @pytest.mark.parametrize("dtype", [C128, C64])
