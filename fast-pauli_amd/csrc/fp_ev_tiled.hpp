@@ -206,37 +206,74 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
     // WIDE = false when 2^n * S < 2^32: offsets fit 32 bits (half the registers).
     typedef typename std::conditional<WIDE, uint64_t, uint32_t>::type Idx;
     bool const s_pow2 = sshift < 64u;
+    // complex64 full tiles with a power-of-two state count and S_blk >= 2 move two
+    // neighbouring states per 16-byte load and LDS store: the thread's m-th pair is
+    // elements 2 (tid + m * kThreads) and the one after (eoff[m]: its 16-byte index)
+    typedef typename fpk::VecOf<float, 2>::type V4;
+    constexpr uint32_t EPT2 = EPT / 2;
+    bool const vec2 = sizeof(V) == 8 && full && s_pow2 && lsb >= 1;
     Idx eoff[EPT];
-#pragma unroll
-    for (uint32_t m = 0; m < EPT; ++m)
+    if (vec2)
     {
-        uint32_t const e = tid + m * kThreads;
-        uint32_t const a = e >> lsb;
-        bool const live = e < tile_elems && t_live;
-        uint64_t const roff = live ? (bt[a >> k] ^ uint64_t(a & lowmask)) : 0;
-        eoff[m] = Idx(s_pow2 ? ((roff << sshift) | (s0 + t)) : roff);
+#pragma unroll
+        for (uint32_t m = 0; m < EPT2; ++m)
+        {
+            uint32_t const e = 2u * (tid + m * kThreads);
+            uint32_t const a = e >> lsb;
+            uint64_t const roff = bt[a >> k] ^ uint64_t(a & lowmask);
+            eoff[m] = Idx(((roff << sshift) | (s0 + (e & (sblk - 1u)))) >> 1);
+            eoff[EPT2 + m] = 0;
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (uint32_t m = 0; m < EPT; ++m)
+        {
+            uint32_t const e = tid + m * kThreads;
+            uint32_t const a = e >> lsb;
+            bool const live = e < tile_elems && t_live;
+            uint64_t const roff = live ? (bt[a >> k] ^ uint64_t(a & lowmask)) : 0;
+            eoff[m] = Idx(s_pow2 ? ((roff << sshift) | (s0 + t)) : roff);
+        }
     }
 
     V reg[EPT];
-    auto fetch = [&](uint64_t cb) {
+    auto fetch = [&](V(&rg)[EPT], uint64_t cb) {
         Idx const cbs = Idx(s_pow2 ? (cb << sshift) : cb);
         auto index = [&](uint32_t m) {
             return s_pow2 ? Idx(cbs ^ eoff[m]) : Idx((cbs ^ eoff[m]) * Idx(n_states) + Idx(s0 + t));
         };
+        if constexpr (sizeof(V) == 8)
+        {
+            if (vec2)
+            {
+                auto const *st4 = reinterpret_cast<V4 const *>(states);
+                Idx const cbs4 = Idx(cb << (sshift - 1u));
+#pragma unroll
+                for (uint32_t m = 0; m < EPT2; ++m)
+                {
+                    V4 const v = __builtin_nontemporal_load(&st4[cbs4 ^ eoff[m]]);
+                    rg[2 * m] = V{v[0], v[1]};
+                    rg[2 * m + 1] = V{v[2], v[3]};
+                }
+                return;
+            }
+        }
         if (full)
         {
 #pragma unroll
             for (uint32_t m = 0; m < EPT; ++m)
-                reg[m] = __builtin_nontemporal_load(&states[index(m)]);
+                rg[m] = __builtin_nontemporal_load(&states[index(m)]);
         }
         else
         {
 #pragma unroll
             for (uint32_t m = 0; m < EPT; ++m)
             {
-                reg[m] = V(0);
+                rg[m] = V(0);
                 if (tid + m * kThreads < tile_elems && t_live)
-                    reg[m] = __builtin_nontemporal_load(&states[index(m)]);
+                    rg[m] = __builtin_nontemporal_load(&states[index(m)]);
             }
         }
     };
@@ -544,12 +581,23 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
         }
     };
 
-    auto store_tile = [&]() {
+    auto store_tile = [&](V const(&rg)[EPT]) {
+        if constexpr (sizeof(V) == 8)
+        {
+            if (vec2)
+            {
+                auto *tile4 = reinterpret_cast<V4 *>(smem);
+#pragma unroll
+                for (uint32_t m = 0; m < EPT2; ++m)
+                    tile4[tid + m * kThreads] = V4{rg[2 * m][0], rg[2 * m][1], rg[2 * m + 1][0], rg[2 * m + 1][1]};
+                return;
+            }
+        }
         if (full)
         {
 #pragma unroll
             for (uint32_t m = 0; m < EPT; ++m)
-                tile[tid + m * kThreads] = reg[m];
+                tile[tid + m * kThreads] = rg[m];
         }
         else
         {
@@ -558,15 +606,16 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
             {
                 uint32_t const e = tid + m * kThreads;
                 if (e < tile_elems)
-                    tile[e] = reg[m];
+                    tile[e] = rg[m];
             }
         }
     };
+    auto next_base = [&](uint64_t b) { return ((b | ~cmask) + cstep) & cmask; };
     uint64_t cb = 0;
     if (q0 < n_cosets)
     {
         cb = coset_base(q0, k, grp);
-        fetch(cb);
+        fetch(reg, cb);
     }
     if constexpr (HOIST)
     {
@@ -577,17 +626,53 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
         Fast fh{};
         if (!idle)
             fh = make_fast(make_unit(wave, lg, 4u), lg, PAIRED ? 16u : kEsz, 4u);
-        for (uint64_t q = q0; q < n_cosets; q += q_step)
-        {
-            __syncthreads(); // the previous tile is no longer read
-            store_tile();
-            __syncthreads();
-            uint64_t const cbn = ((cb | ~cmask) + cstep) & cmask;
-            if (q + q_step < n_cosets)
-                fetch(cbn);
+        auto compute = [&](uint64_t q) {
             if (!idle)
                 run_fast_any(fh, q, lg, std::integral_constant<bool, PAIRED>{});
-            cb = cbn;
+        };
+        // complex128 with 32-bit offsets and 64 KiB tiles has the registers for a second tile in
+        // flight: tiles j + 1 and j + 2 load while tile j is evaluated
+        constexpr bool DEEP = sizeof(V) == 16 && !WIDE && TKB == 64;
+        if constexpr (DEEP)
+        {
+            V reg2[EPT];
+            uint64_t nb = next_base(cb); // base of the tile after the two in flight
+            if (q0 + q_step < n_cosets)
+                fetch(reg2, nb);
+            nb = next_base(nb);
+            for (uint64_t q = q0; q < n_cosets; q += 2 * q_step)
+            {
+                __syncthreads(); // the previous tile is no longer read
+                store_tile(reg);
+                __syncthreads();
+                if (q + 2 * q_step < n_cosets)
+                    fetch(reg, nb);
+                compute(q);
+                if (q + q_step >= n_cosets)
+                    break;
+                uint64_t const nb3 = next_base(nb);
+                __syncthreads();
+                store_tile(reg2);
+                __syncthreads();
+                if (q + 3 * q_step < n_cosets)
+                    fetch(reg2, nb3);
+                compute(q + q_step);
+                nb = next_base(nb3);
+            }
+        }
+        else
+        {
+            for (uint64_t q = q0; q < n_cosets; q += q_step)
+            {
+                __syncthreads(); // the previous tile is no longer read
+                store_tile(reg);
+                __syncthreads();
+                uint64_t const cbn = next_base(cb);
+                if (q + q_step < n_cosets)
+                    fetch(reg, cbn);
+                compute(q);
+                cb = cbn;
+            }
         }
     }
     else
@@ -595,11 +680,11 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
         for (uint64_t q = q0; q < n_cosets; q += q_step)
         {
             __syncthreads(); // the previous tile is no longer read
-            store_tile();
+            store_tile(reg);
             __syncthreads();
-            uint64_t const cbn = ((cb | ~cmask) + cstep) & cmask;
+            uint64_t const cbn = next_base(cb);
             if (q + q_step < n_cosets)
-                fetch(cbn);
+                fetch(reg, cbn);
             for (uint32_t u = wave; u < n_units; u += W)
             {
                 Unit const un = make_unit(u, plain, U1);
