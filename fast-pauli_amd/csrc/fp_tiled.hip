@@ -548,10 +548,10 @@ fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *stat
     uint32_t const chunks_y = uint32_t((n_states + (size_t(1) << geo.lsb) - 1) >> geo.lsb);
     if (chunks_y > 65535u)
         return fail(FP_ERR_INVALID_ARGUMENT, "too many states for the tiled expectation kernel");
-    // up to 8192 blocks (FAST_PAULI_TILE_BLOCKS) with at least ~16 tiles each; the
+    // up to 16384 blocks (FAST_PAULI_TILE_BLOCKS) with at least ~16 tiles each; the
     // `ranges` blocks of a group take every ranges-th coset
     uint64_t const tiles = uint64_t(tt->n_groups) * chunks_y * n_cosets;
-    uint64_t const target = std::max<uint64_t>(256, std::min<uint64_t>(env_u32("FAST_PAULI_TILE_BLOCKS", 8192), tiles / 16));
+    uint64_t const target = std::max<uint64_t>(256, std::min<uint64_t>(env_u32("FAST_PAULI_TILE_BLOCKS", 16384), tiles / 16));
     uint64_t want = (target + uint64_t(tt->n_groups) * chunks_y - 1) / (uint64_t(tt->n_groups) * chunks_y);
     want = std::max<uint64_t>(1, std::min<uint64_t>(want, n_cosets));
     uint32_t const ranges = uint32_t(std::min<uint64_t>(want, 0x7fffffffu));
