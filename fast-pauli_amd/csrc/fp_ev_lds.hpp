@@ -7,17 +7,25 @@
 // of once per string.
 //
 // Strings are grouped on the host into chunks of up to kChunk strings sharing
-// the same x mask, so the two LDS reads and the product w = conj(psi_j) psi_{j^x}
-// of a row serve every string of the chunk.  Rows are enumerated as
-// j = u * 64 + lane with u uniform across the wave, which splits the sign
-// (-1)^{popcount(j & z)} into a per-lane bit (low 6 bits of z, fixed for the
-// chunk) and a wave-uniform bit (popcount(u & z >> 6), scalar ALU).
-//   mode kAllRows: every row j contributes eps * sign * G(w_j), G = Re w (nY even)
-//                  or Im w (nY odd), eps = +1 for nY = 0, 1 mod 4, -1 otherwise;
-//                  used when x = 0 or the highest set bit of x is below 6.
-//   mode kPairs:   rows with bit h of j clear contribute 2 eps sign G(w_j) (the
-//                  partner row's term is the same); used when h >= 6, where the
-//                  leader rows are whole 64-row blocks u with bit h-6 clear.
+// the same x mask and the same half of w (Re w for even nY, Im w for odd nY),
+// so the LDS reads and the product w = conj(psi_j) psi_{j^x} of a row serve every
+// string of the chunk, and only the needed half of w is computed.  Rows are
+// enumerated as j = u * 64 + lane with u uniform across the wave, which splits
+// the sign (-1)^{popcount(j & z)} into a per-lane bit (fixed for the chunk) and a
+// wave-uniform bit (popcount(u & z >> 6), scalar ALU).
+//   mode kAllRows:   every row j contributes eps sign G(w_j); with x = 0 that is
+//                    sign * |psi_j|^2 (no partner read); used for x = 0, and for
+//                    h < 6 when there is only one 64-row block.
+//   mode kPairs:     h >= 6 (h: highest set bit of x): rows with bit h of j clear
+//                    contribute 2 eps sign G(w_j) (the partner row's term is the
+//                    same), G = Re w or Im w, eps = +1 for nY = 0, 1 mod 4, -1
+//                    otherwise; the leader rows are whole 64-row blocks u with
+//                    bit h-6 clear.
+//   mode kPairsLane: 0 < h < 6 (partner in the same 64-row block): iteration v
+//                    covers blocks 2v and 2v + 1, and lane l takes row
+//                    (2v + bit_h(l)) * 64 + l, the member of its pair whose bit h
+//                    equals bit 6 of the row; each pair is visited once and every
+//                    lane has a pair in every iteration.
 // Per-lane row sums use RowAcc<T> (FP32 for complex64); the wave reduction and
 // the final scale are done in the same precision and written to ev(s, t).
 #pragma once
@@ -37,18 +45,33 @@ constexpr int kThreads = 1024;
 enum : uint32_t
 {
     kAllRows = 0,
-    kPairs = 1
+    kPairs = 1,
+    kPairsLane = 2
 };
 
 struct EvChunk
 {
     uint32_t x;     // x mask (n <= 14: fits 32 bits)
-    uint32_t h;     // highest set bit of x (kPairs only)
-    uint32_t mode;  // kAllRows / kPairs
+    uint32_t h;     // highest set bit of x (pair modes)
+    uint32_t mode;  // kAllRows / kPairs / kPairsLane
     uint32_t count; // strings in this chunk (1..kChunk)
     uint32_t first; // index of the first EvStr
-    uint32_t pad[3];
+    uint32_t imag;  // 1: every string of the chunk takes Im w (odd nY), 0: Re w
+    uint32_t pad[2];
 };
+
+// Mode of a string with x mask x (highest set bit h) over `rows` rows.
+__host__ __device__ constexpr uint32_t ev_mode(uint64_t x, uint32_t h, uint64_t rows)
+{
+    return x == 0 ? kAllRows : (h >= 6 ? kPairs : (rows >= 128 ? kPairsLane : kAllRows));
+}
+
+// Row offset of lane `lane` within a pair of 64-row blocks (kPairsLane), or
+// within one block (other modes); the lane's sign bit is parity(offset & z & 127).
+__device__ __forceinline__ uint32_t lane_row(uint32_t mode, uint32_t h, uint32_t lane)
+{
+    return mode == kPairsLane ? lane + (((lane >> h) & 1u) << 6) : lane;
+}
 
 struct EvStr
 {
@@ -100,10 +123,7 @@ __global__ __launch_bounds__(256) void k_transpose(V const *__restrict__ in, V *
     }
 }
 
-// Rows of one chunk with exactly CNT strings.  The wave-uniform row sign bit
-// (popcount(u & zhi) parity) is applied as a scalar +-1 factor; the per-lane bit
-// is applied by the caller once per chunk.  Re w / Im w per string is a select
-// on a wave-uniform flag.
+// v with its sign bit flipped when bit31 = 0x80000000 (exact negation)
 template <class T> __device__ __forceinline__ T xor_sign(T v, uint32_t bit31)
 {
     if constexpr (sizeof(T) == 4)
@@ -112,25 +132,40 @@ template <class T> __device__ __forceinline__ T xor_sign(T v, uint32_t bit31)
         return __longlong_as_double(__double_as_longlong(v) ^ (static_cast<long long>(bit31) << 32));
 }
 
-// Rows of one chunk with exactly CNT strings.  Iterations q = q0 | k (q0 a
-// multiple of U, k < U) visit row blocks u(q) = insert(q) = insert(q0) | insert(k)
+// Rows of one chunk with exactly CNT strings, all taking G = |psi|^2 (KIND 0,
+// x = 0), Re w (KIND 1) or Im w (KIND 2).  Iterations q = q0 | k (q0 a multiple
+// of U, k < U) visit row blocks u(q) = insert(q) = insert(q0) | insert(k)
 // (insert_zero_bit at hh is linear over GF(2)), so the wave-uniform sign bit
 // parity(u & zhi) splits into a per-chunk constant per (string, k), held as a
 // +-1 factor, and one flip bit per (string, q0).  The per-lane bit is applied by
-// the caller once per chunk.  Re w / Im w per string is a select on a
-// wave-uniform flag.
-template <class T, int U, int CNT>
+// the caller once per chunk.
+template <class T, int U, int CNT, int KIND>
 __device__ __forceinline__ void chunk_rows_n(typename fpk::VecOf<T, 1>::type const *col, uint32_t lane,
                                              EvChunk const &ch, uint32_t n_blocks64, uint32_t const (&zhi)[kChunk],
-                                             bool const (&imag)[kChunk], typename fpk::RowAcc<T>::type (&acc)[kChunk])
+                                             typename fpk::RowAcc<T>::type (&acc)[kChunk])
 {
     typedef typename fpk::RowAcc<T>::type A;
+    typedef typename fpk::VecOf<T, 1>::type V;
     uint32_t const x = ch.x;
-    bool const pairs = ch.mode == kPairs;
-    uint32_t const hh = pairs ? ch.h - 6 : 0;
+    bool const pairs = ch.mode != kAllRows;
+    uint32_t const hh = ch.mode == kPairs ? ch.h - 6 : 0;
     uint32_t const lo_mask = (1u << hh) - 1;
     uint32_t const n_iter = pairs ? (n_blocks64 >> 1) : n_blocks64;
+    uint32_t const lr = lane_row(ch.mode, ch.h, lane);
     auto ins = [&](uint32_t q) { return pairs ? (((q >> hh) << (hh + 1)) | (q & lo_mask)) : q; };
+    auto g_of = [&](V const &a, V const &b) {
+        A const ar = A(a[0]), ai = A(a[1]);
+        if constexpr (KIND == 0)
+            return fma(ar, ar, ai * ai);
+        else
+        {
+            A const br = A(b[0]), bi = A(b[1]);
+            if constexpr (KIND == 1)
+                return fma(ar, br, ai * bi);
+            else
+                return fma(ar, bi, -(ai * br));
+        }
+    };
     A sgk[CNT][U];
 #pragma unroll
     for (int s = 0; s < CNT; ++s)
@@ -141,13 +176,16 @@ __device__ __forceinline__ void chunk_rows_n(typename fpk::VecOf<T, 1>::type con
     for (uint32_t q0 = 0; q0 < n_full; q0 += U)
     {
         uint32_t const uq0 = ins(q0);
-        typename fpk::VecOf<T, 1>::type a[U], b[U];
+        V a[U], b[U];
 #pragma unroll
         for (int k = 0; k < U; ++k)
         {
-            uint32_t const j = ((uq0 | ins(uint32_t(k))) << 6) | lane;
+            uint32_t const j = ((uq0 | ins(uint32_t(k))) << 6) + lr;
             a[k] = col[j];
-            b[k] = col[j ^ x];
+            if constexpr (KIND != 0)
+                b[k] = col[j ^ x];
+            else
+                b[k] = a[k];
         }
         uint32_t flip[CNT];
 #pragma unroll
@@ -156,47 +194,57 @@ __device__ __forceinline__ void chunk_rows_n(typename fpk::VecOf<T, 1>::type con
 #pragma unroll
         for (int k = 0; k < U; ++k)
         {
-            A const ar = A(a[k][0]), ai = A(a[k][1]), br = A(b[k][0]), bi = A(b[k][1]);
-            A const gre = fma(ar, br, ai * bi);
-            A const gim = fma(ar, bi, -(ai * br));
+            A const g = g_of(a[k], b[k]);
 #pragma unroll
             for (int s = 0; s < CNT; ++s)
-                acc[s] = fma(xor_sign(sgk[s][k], flip[s]), imag[s] ? gim : gre, acc[s]);
+                acc[s] = fma(xor_sign(sgk[s][k], flip[s]), g, acc[s]);
         }
     }
     for (uint32_t q = n_full; q < n_iter; ++q) // fewer than U row blocks left
     {
         uint32_t const u = ins(q);
-        uint32_t const j = (u << 6) | lane;
-        typename fpk::VecOf<T, 1>::type const av = col[j], bv = col[j ^ x];
-        A const ar = A(av[0]), ai = A(av[1]), br = A(bv[0]), bi = A(bv[1]);
-        A const gre = fma(ar, br, ai * bi);
-        A const gim = fma(ar, bi, -(ai * br));
+        uint32_t const j = (u << 6) + lr;
+        V const av = col[j];
+        V const bv = KIND != 0 ? col[j ^ x] : av;
+        A const g = g_of(av, bv);
 #pragma unroll
         for (int s = 0; s < CNT; ++s)
         {
             A const sg = (__popc(u & zhi[s]) & 1) ? A(-1) : A(1);
-            acc[s] = fma(sg, imag[s] ? gim : gre, acc[s]);
+            acc[s] = fma(sg, g, acc[s]);
         }
+    }
+}
+
+template <class T, int U, int KIND>
+__device__ __forceinline__ void chunk_rows_k(typename fpk::VecOf<T, 1>::type const *col, uint32_t lane,
+                                             EvChunk const &ch, uint32_t n_blocks64, uint32_t const (&zhi)[kChunk],
+                                             typename fpk::RowAcc<T>::type (&acc)[kChunk])
+{
+    switch (ch.count)
+    {
+    case 1: chunk_rows_n<T, U, 1, KIND>(col, lane, ch, n_blocks64, zhi, acc); break;
+    case 2: chunk_rows_n<T, U, 2, KIND>(col, lane, ch, n_blocks64, zhi, acc); break;
+    case 3: chunk_rows_n<T, U, 3, KIND>(col, lane, ch, n_blocks64, zhi, acc); break;
+    case 4: chunk_rows_n<T, U, 4, KIND>(col, lane, ch, n_blocks64, zhi, acc); break;
+    case 5: chunk_rows_n<T, U, 5, KIND>(col, lane, ch, n_blocks64, zhi, acc); break;
+    case 6: chunk_rows_n<T, U, 6, KIND>(col, lane, ch, n_blocks64, zhi, acc); break;
+    case 7: chunk_rows_n<T, U, 7, KIND>(col, lane, ch, n_blocks64, zhi, acc); break;
+    default: chunk_rows_n<T, U, kChunk, KIND>(col, lane, ch, n_blocks64, zhi, acc); break;
     }
 }
 
 template <class T, int U>
 __device__ __forceinline__ void chunk_rows(typename fpk::VecOf<T, 1>::type const *col, uint32_t lane, EvChunk const &ch,
                                            uint32_t n_blocks64, uint32_t const (&zhi)[kChunk],
-                                           bool const (&imag)[kChunk], typename fpk::RowAcc<T>::type (&acc)[kChunk])
+                                           typename fpk::RowAcc<T>::type (&acc)[kChunk])
 {
-    switch (ch.count)
-    {
-    case 1: chunk_rows_n<T, U, 1>(col, lane, ch, n_blocks64, zhi, imag, acc); break;
-    case 2: chunk_rows_n<T, U, 2>(col, lane, ch, n_blocks64, zhi, imag, acc); break;
-    case 3: chunk_rows_n<T, U, 3>(col, lane, ch, n_blocks64, zhi, imag, acc); break;
-    case 4: chunk_rows_n<T, U, 4>(col, lane, ch, n_blocks64, zhi, imag, acc); break;
-    case 5: chunk_rows_n<T, U, 5>(col, lane, ch, n_blocks64, zhi, imag, acc); break;
-    case 6: chunk_rows_n<T, U, 6>(col, lane, ch, n_blocks64, zhi, imag, acc); break;
-    case 7: chunk_rows_n<T, U, 7>(col, lane, ch, n_blocks64, zhi, imag, acc); break;
-    default: chunk_rows_n<T, U, kChunk>(col, lane, ch, n_blocks64, zhi, imag, acc); break;
-    }
+    if (ch.x == 0)
+        chunk_rows_k<T, U, 0>(col, lane, ch, n_blocks64, zhi, acc);
+    else if (ch.imag)
+        chunk_rows_k<T, U, 2>(col, lane, ch, n_blocks64, zhi, acc);
+    else
+        chunk_rows_k<T, U, 1>(col, lane, ch, n_blocks64, zhi, acc);
 }
 
 template <class T>
@@ -233,24 +281,22 @@ __global__ __launch_bounds__(kThreads) void k_ev_lds(typename fpk::VecOf<T, 1>::
             break;
         EvChunk const ch = chunks[c];
         uint32_t zhi[kChunk], lsign[kChunk];
-        bool imag[kChunk];
         A acc[kChunk];
+        uint32_t const lr = lane_row(ch.mode, ch.h, lane);
 #pragma unroll
         for (int s = 0; s < kChunk; ++s)
         {
             acc[s] = A(0);
             zhi[s] = 0;
             lsign[s] = 0;
-            imag[s] = false;
             if (s < int(ch.count))
             {
                 EvStr const st = strs[ch.first + s];
                 zhi[s] = st.z >> 6;
-                lsign[s] = uint32_t(__popc(lane & st.z & 63u) & 1) << 31;
-                imag[s] = st.imag != 0;
+                lsign[s] = uint32_t(__popc(lr & st.z & 127u) & 1) << 31;
             }
         }
-        chunk_rows<T, 4>(col, lane, ch, n_blocks64, zhi, imag, acc);
+        chunk_rows<T, 4>(col, lane, ch, n_blocks64, zhi, acc);
 #pragma unroll
         for (int s = 0; s < kChunk; ++s)
         {
@@ -319,24 +365,22 @@ __global__ __launch_bounds__(kThreads) void k_ev_coset(typename fpk::VecOf<T, 1>
             break;
         EvChunk const ch = chunks[c];
         uint32_t zhi[kChunk], lsign[kChunk];
-        bool imag[kChunk];
         A acc[kChunk];
+        uint32_t const lr = lane_row(ch.mode, ch.h, lane);
 #pragma unroll
         for (int s = 0; s < kChunk; ++s)
         {
             acc[s] = A(0);
             zhi[s] = 0;
             lsign[s] = 0;
-            imag[s] = false;
             if (s < int(ch.count))
             {
                 EvStr const st = strs[ch.first + s];
                 zhi[s] = st.z >> 6;
-                lsign[s] = uint32_t(__popc(lane & st.z & 63u) & 1) << 31;
-                imag[s] = st.imag != 0;
+                lsign[s] = uint32_t(__popc(lr & st.z & 127u) & 1) << 31;
             }
         }
-        chunk_rows<T, 4>(col, lane, ch, n_blocks64, zhi, imag, acc);
+        chunk_rows<T, 4>(col, lane, ch, n_blocks64, zhi, acc);
 #pragma unroll
         for (int s = 0; s < kChunk; ++s)
         {

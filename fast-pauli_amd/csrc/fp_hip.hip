@@ -233,15 +233,17 @@ void fill_coeff_tables(fp_hip_ops *o, ArenaLayout const &L)
     }
 }
 
-// Groups strings with equal x into chunks of at most fpl::kChunk for the LDS
-// expectation kernel, largest work first.  Returns the number of chunks.
+// Groups strings with equal x and equal half of w (the parity of nY) into
+// chunks of at most fpl::kChunk for the LDS expectation kernel, largest work
+// first.  Returns the number of chunks.
 uint32_t build_ev_chunks(fp_hip_ops const *o, fpl::EvChunk *chunks, fpl::EvStr *strs)
 {
     size_t const N = o->n_strings;
     std::vector<uint32_t> order(N);
     for (size_t i = 0; i < N; ++i)
         order[i] = uint32_t(i);
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return o->x[a] < o->x[b]; });
+    auto key = [&](uint32_t a) { return std::make_pair(o->x[a], o->ny[a] & 1u); };
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return key(a) < key(b); });
     struct Pending
     {
         fpl::EvChunk c;
@@ -253,17 +255,18 @@ uint32_t build_ev_chunks(fp_hip_ops const *o, fpl::EvChunk *chunks, fpl::EvStr *
     while (i < N)
     {
         size_t j = i;
-        while (j < N && o->x[order[j]] == o->x[order[i]])
+        while (j < N && key(order[j]) == key(order[i]))
             ++j;
         uint64_t const x = o->x[order[i]];
         uint32_t const h = x ? 63u - uint32_t(__builtin_clzll(x)) : 0u;
-        uint32_t const mode = (x != 0 && h >= 6) ? fpl::kPairs : fpl::kAllRows;
+        uint32_t const mode = fpl::ev_mode(x, h, uint64_t(1) << o->n_qubits);
         for (size_t k = i; k < j; k += fpl::kChunk)
         {
             Pending p{};
             p.c.x = uint32_t(x);
             p.c.h = h;
             p.c.mode = mode;
+            p.c.imag = o->ny[order[i]] & 1u;
             p.c.count = uint32_t(std::min<size_t>(fpl::kChunk, j - k));
             for (size_t m = k; m < k + p.c.count; ++m)
             {
@@ -271,9 +274,9 @@ uint32_t build_ev_chunks(fp_hip_ops const *o, fpl::EvChunk *chunks, fpl::EvStr *
                 uint32_t const ny = o->ny[sidx];
                 double const eps = ((ny & 3u) <= 1u) ? 1.0 : -1.0;
                 p.s.push_back(fpl::EvStr{uint32_t(o->z[sidx]), ny & 1u, sidx, 0u,
-                                         eps * (mode == fpl::kPairs ? 2.0 : 1.0)});
+                                         eps * (mode == fpl::kAllRows ? 1.0 : 2.0)});
             }
-            p.cost = double(p.c.count + 2) * (mode == fpl::kPairs ? 1.0 : 2.0);
+            p.cost = double(p.c.count + 2) * (mode == fpl::kAllRows ? 2.0 : 1.0);
             pend.push_back(std::move(p));
         }
         i = j;

@@ -339,8 +339,8 @@ fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, 
         }
         else
         {
-            // per-string EV matrix: chunks of <= fpl::kChunk strings with equal local x,
-            // costliest first (the LDS kernel's chunk records)
+            // per-string EV matrix: chunks of <= fpl::kChunk strings with equal local x
+            // and equal half of w, costliest first (the LDS kernel's chunk records)
             struct Pending
             {
                 fpl::EvChunk c;
@@ -348,25 +348,30 @@ fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, 
                 double cost;
             };
             std::vector<Pending> pend;
-            for (size_t i = 0; i < hg.mem.size();)
+            std::vector<LocalStr> mem = hg.mem;
+            auto key = [&](LocalStr const &m) { return std::make_pair(m.A, ops->ny[m.s] & 1u); };
+            std::stable_sort(mem.begin(), mem.end(),
+                             [&](LocalStr const &a, LocalStr const &b) { return key(a) < key(b); });
+            for (size_t i = 0; i < mem.size();)
             {
                 size_t j = i;
-                while (j < hg.mem.size() && j - i < size_t(fpl::kChunk) && hg.mem[j].A == hg.mem[i].A)
+                while (j < mem.size() && j - i < size_t(fpl::kChunk) && key(mem[j]) == key(mem[i]))
                     ++j;
                 Pending p{};
-                p.c.x = hg.mem[i].A;
+                p.c.x = mem[i].A;
                 p.c.h = p.c.x ? 31u - uint32_t(__builtin_clz(p.c.x)) : 0u;
-                p.c.mode = (p.c.x != 0 && p.c.h >= 6) ? fpl::kPairs : fpl::kAllRows;
+                p.c.mode = fpl::ev_mode(p.c.x, p.c.h, uint64_t(1) << (k + dh));
+                p.c.imag = ops->ny[mem[i].s] & 1u;
                 p.c.count = uint32_t(j - i);
                 for (size_t m = i; m < j; ++m)
                 {
-                    uint32_t const sidx = hg.mem[m].s;
+                    uint32_t const sidx = mem[m].s;
                     uint32_t const ny = ops->ny[sidx];
                     double const eps = ((ny & 3u) <= 1u) ? 1.0 : -1.0;
-                    p.s.push_back(fpl::EvStr{hg.mem[m].Z, ny & 1u, sidx, uint32_t(hg.mem[m].zc),
-                                             eps * (p.c.mode == fpl::kPairs ? 2.0 : 1.0)});
+                    p.s.push_back(fpl::EvStr{mem[m].Z, ny & 1u, sidx, uint32_t(mem[m].zc),
+                                             eps * (p.c.mode == fpl::kAllRows ? 1.0 : 2.0)});
                 }
-                p.cost = double(p.c.count + 2) * (p.c.mode == fpl::kPairs ? 1.0 : 2.0);
+                p.cost = double(p.c.count + 2) * (p.c.mode == fpl::kAllRows ? 2.0 : 1.0);
                 pend.push_back(std::move(p));
                 i = j;
             }
