@@ -29,6 +29,9 @@
 // HBM traffic is (number of groups) x (state bytes) instead of
 // (number of strings) x (state bytes).  The tile is double-buffered through
 // registers: the next coset is in flight while the current one is evaluated.
+// Each launch shape has two instances: HOIST, for groups whose waves each own
+// at most one single-string unit (set up once per block, complex64 with two
+// states per lane), and the general one, which sets units up per tile.
 // Each block accumulates FP64 sums for its state and its range of cosets and
 // writes one partial per state; partials are reduced in a fixed order, so
 // results are identical run to run.
@@ -149,7 +152,9 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
     constexpr uint32_t EPT = kElems / kThreads;
     constexpr uint32_t kEsz = sizeof(V);
     constexpr int U = 4;
-    constexpr int U1 = sizeof(V) == 16 ? 4 : 8; // passes per group in the one-string fast path
+    // passes per step of the one-string fast path in the per-tile instance (the
+    // set-up-once instance uses 4)
+    constexpr int U1 = sizeof(V) == 16 ? 4 : 8;
     extern __shared__ __align__(16) unsigned char smem[];
     V *tile = reinterpret_cast<V *>(smem);
 
