@@ -15,6 +15,12 @@
 // no fused multiply-add) in the same order: results are IEEE-equal to it.
 // HBM traffic per group is one read of the state plus one read and one write of
 // the output, instead of one state read per string.
+//
+// A thread owns "slots" of the tile: one element, or (PAIRED: complex64 full
+// tiles with a power-of-two state count, S_blk >= 2) the two neighbouring
+// states of one row, moved by 16-byte loads, stores and LDS reads.  Coset bases
+// advance by one deposited addition per tile, and a slot's output values are
+// requested before the tile store, so their latency overlaps it.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -44,8 +50,9 @@ struct ApplyStr
 };
 
 // One group: launch over cosets [0, n_cosets) of group g, blocks striding by
-// gridDim.x; state chunk blockIdx.y.
-template <class T, bool WIDE>
+// gridDim.x; state chunk blockIdx.y.  PAIRED requires complex64, S_blk >= 2, a
+// power-of-two state count that S_blk divides, and 2^d rows >= one full tile.
+template <class T, bool WIDE, bool PAIRED>
 __global__ __launch_bounds__(kThreads, 4) void k_op_apply_tiled(typename fpk::VecOf<T, 1>::type const *__restrict__ states,
                                                              typename fpk::VecOf<T, 1>::type *__restrict__ out,
                                                              uint64_t n_states, uint32_t sshift, uint32_t lsb,
@@ -54,12 +61,17 @@ __global__ __launch_bounds__(kThreads, 4) void k_op_apply_tiled(typename fpk::Ve
                                                              uint64_t const *__restrict__ btab,
                                                              ApplyStr const *__restrict__ strs, int init_zero)
 {
+    static_assert(!PAIRED || sizeof(T) == 4, "two states per slot: complex64 only");
     typedef typename fpk::VecOf<T, 1>::type V;
-    constexpr uint32_t kElems = kTileBytes / sizeof(V);
-    constexpr uint32_t EPT = kElems / kThreads;
-    constexpr uint32_t kEsz = sizeof(V);
+    typedef typename std::conditional<PAIRED, typename fpk::VecOf<float, 2>::type, V>::type LV; // one slot
+    constexpr uint32_t NS = PAIRED ? 2 : 1;                                                   // states per slot
+    constexpr uint32_t kSlots = kTileBytes / sizeof(LV);
+    constexpr uint32_t EPT = kSlots / kThreads;
+    constexpr uint32_t kLsz = sizeof(LV);
     extern __shared__ __align__(16) unsigned char smem[];
-    V *tile = reinterpret_cast<V *>(smem);
+    LV *tile = reinterpret_cast<LV *>(smem);
+    LV const *src_l = reinterpret_cast<LV const *>(states);
+    LV *out_l = reinterpret_cast<LV *>(out);
 
     uint32_t const tid = threadIdx.x;
     fpt::TileGroup const grp = *group;
@@ -70,75 +82,95 @@ __global__ __launch_bounds__(kThreads, 4) void k_op_apply_tiled(typename fpk::Ve
     ApplyStr const *gs = strs + grp.chunk_first;
 
     uint32_t const sblk = 1u << lsb;
-    uint32_t const t = tid & (sblk - 1u);
+    uint32_t const lsbx = PAIRED ? lsb - 1u : lsb; // slots per row: 2^lsbx
+    uint32_t const tx = tid & ((1u << lsbx) - 1u);
+    uint32_t const t0 = PAIRED ? 2u * tx : tx; // first state of this thread's slots within the block
     uint64_t const s0 = uint64_t(blockIdx.y) << lsb;
-    bool const t_live = s0 + t < n_states;
-    uint32_t const tile_elems = (1u << d) << lsb;
+    bool const t_live = s0 + t0 < n_states;
+    uint32_t const tile_slots = (1u << d) << lsbx;
     uint32_t const lowmask = (1u << k) - 1u;
+    // every slot of every tile of this block is live: loads and stores need no guard
+    bool const full = tile_slots == kSlots && s0 + sblk <= n_states;
 
     typedef typename std::conditional<WIDE, uint64_t, uint32_t>::type Idx;
     bool const s_pow2 = sshift < 64u;
+    // slot m is tile slot tid + m * kThreads: local row a_m = (tid >> lsbx) + (m << rm)
     Idx eoff[EPT];
 #pragma unroll
     for (uint32_t m = 0; m < EPT; ++m)
     {
         uint32_t const e = tid + m * kThreads;
-        uint32_t const a = e >> lsb;
-        bool const live = e < tile_elems && t_live;
+        uint32_t const a = e >> lsbx;
+        bool const live = e < tile_slots && t_live;
         uint64_t const roff = live ? (bt[a >> k] ^ uint64_t(a & lowmask)) : 0;
-        eoff[m] = Idx(s_pow2 ? ((roff << sshift) | (s0 + t)) : roff);
+        uint64_t const el = (roff << sshift) | (s0 + t0); // element index, power-of-two S
+        eoff[m] = Idx(s_pow2 ? (PAIRED ? (el >> 1) : el) : roff);
     }
-
+    // slot index of slot m in coset base cb (elements; pairs of elements when PAIRED)
     auto index_of = [&](uint64_t cb, uint32_t m) -> Idx {
-        Idx const cbs = Idx(s_pow2 ? (cb << sshift) : cb);
-        return s_pow2 ? Idx(cbs ^ eoff[m]) : Idx((cbs ^ eoff[m]) * Idx(n_states) + Idx(s0 + t));
+        if constexpr (PAIRED)
+            return Idx(Idx(cb << (sshift - 1u)) ^ eoff[m]);
+        else
+        {
+            Idx const cbs = Idx(s_pow2 ? (cb << sshift) : cb);
+            return s_pow2 ? Idx(cbs ^ eoff[m]) : Idx((cbs ^ eoff[m]) * Idx(n_states) + Idx(s0 + t0));
+        }
     };
+    auto slot_live = [&](uint32_t m) { return full || (tid + m * kThreads < tile_slots && t_live); };
 
-    V reg[EPT];
-    auto fetch = [&](uint64_t q) {
-        uint64_t const cb = fpt::coset_base(q, k, grp);
+    LV reg[EPT];
+    auto fetch = [&](uint64_t cb) {
 #pragma unroll
         for (uint32_t m = 0; m < EPT; ++m)
         {
-            reg[m] = V(0);
-            if (tid + m * kThreads < tile_elems && t_live)
-                reg[m] = __builtin_nontemporal_load(&states[index_of(cb, m)]);
+            reg[m] = LV(0);
+            if (slot_live(m))
+                reg[m] = __builtin_nontemporal_load(&src_l[index_of(cb, m)]);
         }
     };
 
+    // cosets blockIdx.x, + gridDim.x, ...; base(q + step) = ((base(q) | ~cmask) +
+    // base(step)) & cmask: the pivot bits, set to 1, pass carries through
     uint64_t const q_step = gridDim.x;
+    uint64_t const cmask = fpt::coset_base(n_cosets - 1u, k, grp);
+    uint64_t const cstep = fpt::coset_base(q_step, k, grp);
+    uint64_t cb = 0;
     if (blockIdx.x < n_cosets)
-        fetch(blockIdx.x);
+    {
+        cb = fpt::coset_base(blockIdx.x, k, grp);
+        fetch(cb);
+    }
+    // Slot m holds local row a_m = a_lane + (m << rm), a_lane = tid >> lsbx < 2^rm
+    // (disjoint bits).  For a string (A, Z): the partner row a_m ^ A = (a_lane ^
+    // A_lane) + ((m << rm) ^ A_m) and the sign bit parity(a_m & Z) =
+    // parity(a_lane & Z) ^ parity((m << rm) & Z): a per-lane pointer and sign per
+    // string, plus a wave-uniform offset and bit per slot m.
+    uint32_t const rm = 9u - lsbx; // log2(kThreads) - lsbx
+    uint32_t const a_lane = tid >> lsbx;
+    uint32_t const lane_mask = (1u << rm) - 1u;
     for (uint64_t q = blockIdx.x; q < n_cosets; q += q_step)
     {
+        // this coset's output values, requested before the tile store
+        LV acc[EPT];
+#pragma unroll
+        for (uint32_t m = 0; m < EPT; ++m)
+        {
+            acc[m] = LV(0);
+            if (!init_zero && slot_live(m))
+                acc[m] = out_l[index_of(cb, m)];
+        }
         __syncthreads(); // the previous tile is no longer read
 #pragma unroll
         for (uint32_t m = 0; m < EPT; ++m)
         {
             uint32_t const e = tid + m * kThreads;
-            if (e < tile_elems)
+            if (full || e < tile_slots)
                 tile[e] = reg[m];
         }
         __syncthreads();
-        uint64_t const cb = fpt::coset_base(q, k, grp);
+        uint64_t const cbn = ((cb | ~cmask) + cstep) & cmask;
         if (q + q_step < n_cosets)
-            fetch(q + q_step);
-        V acc[EPT];
-#pragma unroll
-        for (uint32_t m = 0; m < EPT; ++m)
-        {
-            acc[m] = V(0);
-            if (!init_zero && tid + m * kThreads < tile_elems && t_live)
-                acc[m] = out[index_of(cb, m)];
-        }
-        // Element m of this thread is local row a_m = a_lane + (m << rm), a_lane =
-        // tid >> lsb < 2^rm (disjoint bits).  For a string (A, Z): the partner row
-        // a_m ^ A = (a_lane ^ A_lane) + ((m << rm) ^ A_m) and the sign bit
-        // parity(a_m & Z) = parity(a_lane & Z) ^ parity((m << rm) & Z): a per-lane
-        // pointer and sign per string, plus a wave-uniform offset and bit per slot m.
-        uint32_t const rm = 9u - lsb; // log2(kThreads) - lsb
-        uint32_t const a_lane = tid >> lsb;
-        uint32_t const lane_mask = (1u << rm) - 1u;
+            fetch(cbn);
         for (uint32_t s = 0; s < n_str; ++s)
         {
             ApplyStr const st = gs[s];
@@ -155,7 +187,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_op_apply_tiled(typename fpk::Ve
                 ci = st.fi;
             }
             uint32_t const A_lane = st.A & lane_mask, A_m = st.A & ~lane_mask;
-            unsigned char const *pl = smem + ((((a_lane ^ A_lane) << lsb) | t) * kEsz);
+            unsigned char const *pl = smem + ((((a_lane ^ A_lane) << lsbx) | tx) * kLsz);
             // lane sign folded into the coefficient (negation is exact)
             bool const lneg = ((uint32_t(__popc(a_lane & st.Z)) ^ cs) & 1u) != 0;
             T const lcr = lneg ? -cr : cr;
@@ -164,20 +196,25 @@ __global__ __launch_bounds__(kThreads, 4) void k_op_apply_tiled(typename fpk::Ve
             for (uint32_t m = 0; m < EPT; ++m)
             {
                 uint32_t const mrow = m << rm;
-                V const src = *reinterpret_cast<V const *>(pl + (((mrow ^ A_m) << lsb) * kEsz));
+                LV const src = *reinterpret_cast<LV const *>(pl + (((mrow ^ A_m) << lsbx) * kLsz));
                 uint32_t const mbit = uint32_t(__popc(mrow & st.Z) & 1u) << 31;
                 T const a = fpt::xor_sign_bit(lcr, mbit);
                 T const b = fpt::xor_sign_bit(lci, mbit);
-                T orr, oi;
-                fpk::cmul_exact<T>(a, b, src[0], src[1], orr, oi);
-                acc[m][0] = fpk::add_exact<T>(acc[m][0], orr);
-                acc[m][1] = fpk::add_exact<T>(acc[m][1], oi);
+#pragma unroll
+                for (uint32_t j = 0; j < NS; ++j)
+                {
+                    T orr, oi;
+                    fpk::cmul_exact<T>(a, b, src[2 * j], src[2 * j + 1], orr, oi);
+                    acc[m][2 * j] = fpk::add_exact<T>(acc[m][2 * j], orr);
+                    acc[m][2 * j + 1] = fpk::add_exact<T>(acc[m][2 * j + 1], oi);
+                }
             }
         }
 #pragma unroll
         for (uint32_t m = 0; m < EPT; ++m)
-            if (tid + m * kThreads < tile_elems && t_live)
-                __builtin_nontemporal_store(acc[m], &out[index_of(cb, m)]);
+            if (slot_live(m))
+                __builtin_nontemporal_store(acc[m], &out_l[index_of(cb, m)]);
+        cb = cbn;
     }
 }
 

@@ -688,13 +688,19 @@ fp_status run_op_apply_tiled(fp_hip_ops const *ops, int dtype, void const *state
                                                                chunks_y));
     uint32_t const sshift = (n_states & (n_states - 1)) == 0 ? log2u(n_states) : 64u;
     bool const wide = (uint64_t(dim) * n_states) >> 32 != 0 || env_u32("FAST_PAULI_TILE_WIDE", 0) == 1;
+    // complex64 full tiles with a power-of-two state count that one block's
+    // states divide: two states per slot (16-byte loads, stores and LDS reads)
+    bool const paired = dtype == FP_C64 && geo.lsb >= 1 && sshift < 64u && n_states % (size_t(1) << geo.lsb) == 0 &&
+                        geo.d + geo.lsb == log2u(fpa::kTileBytes / 8u);
     static std::once_flag attr_once;
     static hipError_t attr_err = hipSuccess;
     std::call_once(attr_once, [] {
-        void const *fns[] = {reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<double, false>),
-                             reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<double, true>),
-                             reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<float, false>),
-                             reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<float, true>)};
+        void const *fns[] = {reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<double, false, false>),
+                             reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<double, true, false>),
+                             reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<float, false, false>),
+                             reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<float, true, false>),
+                             reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<float, false, true>),
+                             reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<float, true, true>)};
         for (void const *f : fns)
         {
             hipError_t const e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, int(fpa::kTileBytes));
@@ -712,7 +718,8 @@ fp_status run_op_apply_tiled(fp_hip_ops const *ops, int dtype, void const *state
         if (dtype == FP_C128)
         {
             typedef fpk::VecOf<double, 1>::type V;
-            auto const kfn = wide ? &fpa::k_op_apply_tiled<double, true> : &fpa::k_op_apply_tiled<double, false>;
+            auto const kfn =
+                wide ? &fpa::k_op_apply_tiled<double, true, false> : &fpa::k_op_apply_tiled<double, false, false>;
             hipLaunchKernelGGL(kfn, grid, dim3(fpa::kThreads), fpa::kTileBytes, st, static_cast<V const *>(states),
                                static_cast<V *>(out), uint64_t(n_states), sshift, geo.lsb, geo.k, geo.d, n_cosets,
                                tt->groups + g, tt->btab, strs, init_zero);
@@ -720,7 +727,10 @@ fp_status run_op_apply_tiled(fp_hip_ops const *ops, int dtype, void const *state
         else
         {
             typedef fpk::VecOf<float, 1>::type V;
-            auto const kfn = wide ? &fpa::k_op_apply_tiled<float, true> : &fpa::k_op_apply_tiled<float, false>;
+            auto const kfn = paired ? (wide ? &fpa::k_op_apply_tiled<float, true, true>
+                                            : &fpa::k_op_apply_tiled<float, false, true>)
+                                    : (wide ? &fpa::k_op_apply_tiled<float, true, false>
+                                            : &fpa::k_op_apply_tiled<float, false, false>);
             hipLaunchKernelGGL(kfn, grid, dim3(fpa::kThreads), fpa::kTileBytes, st, static_cast<V const *>(states),
                                static_cast<V *>(out), uint64_t(n_states), sshift, geo.lsb, geo.k, geo.d, n_cosets,
                                tt->groups + g, tt->btab, strs, init_zero);
