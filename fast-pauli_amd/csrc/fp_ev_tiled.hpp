@@ -29,9 +29,11 @@
 // HBM traffic is (number of groups) x (state bytes) instead of
 // (number of strings) x (state bytes).  The tile is double-buffered through
 // registers: the next coset is in flight while the current one is evaluated.
-// Each launch shape has two instances: HOIST, for groups whose waves each own
-// at most one single-string unit (set up once per block, complex64 with two
-// states per lane), and the general one, which sets units up per tile.
+// Each launch shape has three instances: shared leaders, for groups whose
+// strings all pair rows with local bit k clear (one leader read serves every
+// string); set-up-once, for groups whose waves each own at most one
+// single-string unit; and the general one, which sets units up per tile.
+// Modes 1 and 2 give complex64 lanes two states each.
 // Each block accumulates FP64 sums for its state and its range of cosets and
 // writes one partial per state; partials are reduced in a fixed order, so
 // results are identical run to run.
@@ -63,10 +65,35 @@ struct TileGroup
     uint32_t btab_off; // first entry of this group's B table (2^dh entries)
     uint64_t piv_lo;   // pivot bit positions 0..7, ascending, 8 bits each
     uint64_t piv_hi;   // pivot bit positions 8..15
-    uint32_t hoist;    // bit hoist_bit(W, U1, r): every wave of a block has at most one
-                       // unit and it takes the one-string fast path (expectation tables)
-    uint32_t pad;
+    uint32_t hoist;     // bit hoist_bit(W, U1, r): every wave of a block has at most one
+                        // unit and it takes the one-string fast path (expectation tables)
+    uint32_t shared;    // bit shared_bit(W, c64, rr): the group runs the shared-leader
+                        // form (expectation tables)
+    uint32_t str_first; // first ShStr of the group (expectation tables)
+    uint32_t n_str;     // strings of the group
 };
+
+// Strings of a shared-leader group: every local x mask has bit k set, so the
+// leader rows (local bit k clear) are common to all of them.
+constexpr int kMaxShared = 8;
+
+struct ShStr
+{
+    uint64_t zc;   // z compressed onto the coset-index bits: coset sign = parity(q & zc)
+    uint32_t A;    // local x mask (bit k set)
+    uint32_t Z;    // local z mask
+    uint32_t imag; // 1: Im w (odd nY), 0: Re w
+    uint32_t pad;
+    double sre; // scale c_s * kappa_s
+    double sim;
+};
+
+// Bit of TileGroup::shared for W waves per block, complex64 (two states per
+// lane) or not, and 2^rr leader rows per pass.
+__host__ __device__ constexpr uint32_t shared_bit(uint32_t W, bool c64, uint32_t rr)
+{
+    return (W == 16 ? 16u : 0u) + (c64 ? 8u : 0u) + rr;
+}
 
 // Bit of TileGroup::hoist for W waves per block, U1 passes per fast-path step and
 // 2^r leader rows per pass (r = 6 - log2 S_blk).
@@ -132,7 +159,7 @@ __device__ __forceinline__ uint64_t coset_base(uint64_t q, uint32_t k, TileGroup
 // owns a single one-string unit keeps its addresses, signs and scale in
 // registers for all of the block's cosets, and coset bases advance by one
 // deposited addition per tile (see below) instead of a pivot-insertion loop.
-template <class T, bool WIDE, int TKB, bool HOIST>
+template <class T, bool WIDE, int TKB, int MODE>
 __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typename fpk::VecOf<T, 1>::type const *__restrict__ states,
                                                               uint64_t n_states, uint32_t sshift, uint32_t lsb,
                                                               uint32_t k, uint32_t d, uint32_t ranges,
@@ -142,6 +169,7 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
                                                               uint64_t const *__restrict__ btab,
                                                               TileChunk const *__restrict__ chunks,
                                                               TileStr const *__restrict__ strs,
+                                                              ShStr const *__restrict__ sh,
                                                               fpk::Dbl2 *__restrict__ partial)
 {
     typedef typename fpk::VecOf<T, 1>::type V;
@@ -173,14 +201,20 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
     uint64_t const *bt = btab + grp.btab_off;
     constexpr uint32_t W = kThreads / 64;
     uint32_t const r = 6u - lsb;
-    // Groups whose waves set their unit up once run in the HOIST instance, the
-    // others in the general one (a launch may need both; each block runs in one).
-    // HOIST runs 4 passes per step, complex64 in the paired lane form (2^(r+1)
-    // leader rows per pass), which needs S_blk >= 2.
-    constexpr bool PAIRED = HOIST && sizeof(V) == 8;
-    bool const g_hoist = (sizeof(V) != 8 || lsb >= 1) &&
-                         ((grp.hoist >> hoist_bit(W, 4u, sizeof(V) == 8 ? r + 1u : r)) & 1u) != 0;
-    if (g_hoist != HOIST)
+    // Instances (MODE): 2 shared leaders (TileGroup::shared), 1 set up once
+    // (TileGroup::hoist), 0 general; a group runs in the first that applies, and a
+    // launch may need several (each block runs in one).  Modes 1 and 2 run 4
+    // passes per step, complex64 in the paired lane form (2^(r+1) leader rows per
+    // pass), which needs S_blk >= 2.
+    static_assert(MODE != 2 || sizeof(T) == 8, "the shared-leader instance is complex128 only");
+    constexpr bool HOIST = MODE == 1;
+    constexpr bool PAIRED = MODE >= 1 && sizeof(V) == 8;
+    uint32_t const rrx = sizeof(V) == 8 ? r + 1u : r;
+    bool const can_pair = sizeof(V) != 8 || lsb >= 1;
+    bool const g_shared = can_pair && ((grp.shared >> shared_bit(W, sizeof(V) == 8, rrx)) & 1u) != 0;
+    bool const g_hoist = !g_shared && can_pair && ((grp.hoist >> hoist_bit(W, 4u, rrx)) & 1u) != 0;
+    int const gmode = g_shared ? 2 : (g_hoist ? 1 : 0);
+    if (gmode != MODE)
         return; // whole block, before any barrier
 
     uint32_t const sblk = 1u << lsb;
@@ -622,6 +656,20 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
         cb = coset_base(q0, k, grp);
         fetch(reg, cb);
     }
+    // tiles q0, q0 + q_step, ... with the next tile in flight during `compute`
+    auto tile_loop = [&](auto &&compute) {
+        for (uint64_t q = q0; q < n_cosets; q += q_step)
+        {
+            __syncthreads(); // the previous tile is no longer read
+            store_tile(reg);
+            __syncthreads();
+            uint64_t const cbn = next_base(cb);
+            if (q + q_step < n_cosets)
+                fetch(reg, cbn);
+            compute(q);
+            cb = cbn;
+        }
+    };
     if constexpr (HOIST)
     {
         // every wave has at most one unit, on the fast path: set it up once
@@ -666,30 +714,149 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
             }
         }
         else
+            tile_loop(compute);
+    }
+    else if constexpr (MODE == 2)
+    {
+        // Shared leaders: every string of the group has local bit k set, so the
+        // rows with bit k clear lead the pairs of all of them.  Wave w takes passes
+        // it = 4w + i (i < 4) of the 2^(d-1) leader rows, 2^rr rows per pass, for
+        // every string: one LDS read per leader row and one per (string, leader row).
+        //   leader  a_i = af + ((4w + i) << (rr + 1)),  af = insert_zero_bit(pf, k)
+        //   partner a_i ^ A = (af ^ A_l) + ((4w << (rr + 1)) ^ A_w) + ((i ^ ci) << (rr + 1))
+        // with A_l, ci, A_w the bits of A in the lane, pass and wave parts (disjoint),
+        // so both byte addresses are a per-lane base plus (i or i ^ ci) * 128 * ESZ,
+        // and the sign parity(a_i & Z) is a per-lane bit (applied once per block)
+        // XOR parity(i & zi), zi = the pass part of Z.  k < rr (host condition).
+        typedef typename std::conditional<PAIRED, typename fpk::VecOf<float, 2>::type, V>::type LV;
+        constexpr uint32_t ESZ = sizeof(LV);
+        constexpr int NS = PAIRED ? 2 : 1;    // states per lane
+        constexpr uint32_t STRIDE = 128u * ESZ; // one pass: 2^(rr + 1 + lsbx) * ESZ bytes, rr + lsbx = 6
+        uint32_t const sb2 = PAIRED ? (sblk >> 1) : sblk;
+        Lanes const lg = PAIRED ? Lanes{lsb - 1u, lane & (sb2 - 1u), r + 1u, lane >> (lsb - 1u)} : plain;
+        uint32_t const rr = lg.rr;
+        uint32_t const n_sh = grp.n_str; // <= kMaxShared
+        ShStr const *ss = sh + grp.str_first;
+        uint32_t const af = insert_zero_bit32(lg.pf, k);
+        uint32_t const wpart = (wave * 4u) << (rr + 1u);
+        uint32_t const la0 = (((af + wpart) << lg.lsbx) | lg.tx) * ESZ;
+        uint32_t const lmask = (1u << (rr + 1u)) - 1u;
+        uint32_t const wmask = ~((1u << (rr + 3u)) - 1u);
+        uint32_t pb[kMaxShared];
+        uint32_t neg_bits = 0; // bit s: parity((af + wpart) & Z_s)
+        // per string, 4 bits each: case (Im flag * 4 + ci) in bits 0..2 of a
+        // nibble of `cases`, and zi in the matching nibble of `zis`
+        uint32_t cases = 0, zis = 0;
+        Acc acc_s[kMaxShared][NS];
+#pragma unroll
+        for (int s = 0; s < kMaxShared; ++s)
         {
-            for (uint64_t q = q0; q < n_cosets; q += q_step)
+            pb[s] = la0;
+#pragma unroll
+            for (int n = 0; n < NS; ++n)
+                acc_s[s][n] = Acc(0);
+            if (s < int(n_sh))
             {
-                __syncthreads(); // the previous tile is no longer read
-                store_tile(reg);
-                __syncthreads();
-                uint64_t const cbn = next_base(cb);
-                if (q + q_step < n_cosets)
-                    fetch(reg, cbn);
-                compute(q);
-                cb = cbn;
+                uint32_t const A = ss[s].A, Z = ss[s].Z;
+                pb[s] = ((((af ^ (A & lmask)) + (wpart ^ (A & wmask))) << lg.lsbx) | lg.tx) * ESZ;
+                neg_bits |= uint32_t(__popc((af + wpart) & Z) & 1) << s;
+                cases |= (((ss[s].imag ? 4u : 0u) + ((A >> (rr + 1u)) & 3u)) << (4 * s));
+                zis |= ((Z >> (rr + 1u)) & 3u) << (4 * s);
+            }
+        }
+        cases = __builtin_amdgcn_readfirstlane(cases);
+        zis = __builtin_amdgcn_readfirstlane(zis);
+        auto compute = [&](uint64_t q) {
+            // coset signs of every string, from the scalar cache before the LDS reads
+            uint32_t cneg = 0;
+#pragma unroll
+            for (int s = 0; s < kMaxShared; ++s)
+                if (s < int(n_sh))
+                    cneg |= uint32_t(__popcll(q & ss[s].zc) & 1) << s;
+            LV L[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                L[i] = *reinterpret_cast<LV const *>(smem + (la0 + uint32_t(i) * STRIDE));
+#pragma unroll
+            for (int s = 0; s < kMaxShared; ++s)
+            {
+                if (s < int(n_sh))
+                {
+                    uint32_t const cse = (cases >> (4 * s)) & 7u;
+                    uint32_t const zi = (zis >> (4 * s)) & 3u;
+                    Acc const s1 = (zi & 1u) ? Acc(-1) : Acc(1);
+                    Acc const s2 = (zi & 2u) ? Acc(-1) : Acc(1);
+                    Acc const cs = ((cneg >> s) & 1u) ? Acc(-1) : Acc(1);
+                    LV R[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        R[j] = *reinterpret_cast<LV const *>(smem + (pb[s] + uint32_t(j) * STRIDE));
+                    // sum_i (-1)^{parity(i & zi)} G(L[i], R[i ^ ci]) per state
+                    auto body = [&](auto im_tag, auto ci_tag) {
+                        constexpr bool IM = decltype(im_tag)::value;
+                        constexpr int CI = decltype(ci_tag)::value;
+#pragma unroll
+                        for (int n = 0; n < NS; ++n)
+                        {
+                            Acc G[4];
+#pragma unroll
+                            for (int i = 0; i < 4; ++i)
+                            {
+                                Acc const lr = Acc(L[i][2 * n]), li = Acc(L[i][2 * n + 1]);
+                                Acc const pr = Acc(R[i ^ CI][2 * n]), pi = Acc(R[i ^ CI][2 * n + 1]);
+                                G[i] = IM ? fma(lr, pi, -(li * pr)) : fma(lr, pr, li * pi);
+                            }
+                            Acc const t0 = fma(s1, G[1], G[0]);
+                            Acc const t1 = fma(s1, G[3], G[2]);
+                            acc_s[s][n] = fma(cs, fma(s2, t1, t0), acc_s[s][n]);
+                        }
+                    };
+                    typedef std::true_type Im;
+                    typedef std::false_type Re;
+                    switch (cse)
+                    {
+                    case 0: body(Re{}, std::integral_constant<int, 0>{}); break;
+                    case 1: body(Re{}, std::integral_constant<int, 1>{}); break;
+                    case 2: body(Re{}, std::integral_constant<int, 2>{}); break;
+                    case 3: body(Re{}, std::integral_constant<int, 3>{}); break;
+                    case 4: body(Im{}, std::integral_constant<int, 0>{}); break;
+                    case 5: body(Im{}, std::integral_constant<int, 1>{}); break;
+                    case 6: body(Im{}, std::integral_constant<int, 2>{}); break;
+                    default: body(Im{}, std::integral_constant<int, 3>{}); break;
+                    }
+                }
+            }
+        };
+        tile_loop(compute);
+        // per-lane sign and string scale, once per block
+#pragma unroll
+        for (int s = 0; s < kMaxShared; ++s)
+        {
+            if (s < int(n_sh))
+            {
+                double const sre = ss[s].sre, sim = ss[s].sim;
+                bool const neg = ((neg_bits >> s) & 1u) != 0;
+#pragma unroll
+                for (int n = 0; n < NS; ++n)
+                {
+                    double const v = neg ? -double(acc_s[s][n]) : double(acc_s[s][n]);
+                    if (n == 0)
+                    {
+                        acc_re += sre * v;
+                        acc_im += sim * v;
+                    }
+                    else
+                    {
+                        acc_re1 += sre * v;
+                        acc_im1 += sim * v;
+                    }
+                }
             }
         }
     }
     else
     {
-        for (uint64_t q = q0; q < n_cosets; q += q_step)
-        {
-            __syncthreads(); // the previous tile is no longer read
-            store_tile(reg);
-            __syncthreads();
-            uint64_t const cbn = next_base(cb);
-            if (q + q_step < n_cosets)
-                fetch(reg, cbn);
+        tile_loop([&](uint64_t q) {
             for (uint32_t u = wave; u < n_units; u += W)
             {
                 Unit const un = make_unit(u, plain, U1);
@@ -698,8 +865,7 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
                 else
                     run_general(un, q);
             }
-            cb = cbn;
-        }
+        });
     }
     // block sum per state, in a fixed order
     __syncthreads();

@@ -66,11 +66,10 @@ struct TileTables
 {
     int kind = 0;
     uint32_t k = 0, dh = 0, n_groups = 0, max_chunks = 0;
-    uint32_t hoist_all = 0, hoist_any = 0; // AND / OR of the groups' TileGroup::hoist bits
     bool valid = false;     // built for the current strings and coefficients
     bool on_device = false; // the device copy matches the host image
     std::vector<unsigned char> h;
-    size_t o_btab = 0, o_chunks = 0, o_strs = 0;
+    size_t o_btab = 0, o_chunks = 0, o_strs = 0, o_sh = 0;
     void *d_mem = nullptr;
     size_t cap = 0;
     hipEvent_t uploaded = nullptr; // completes when the last upload has read `h`
@@ -78,6 +77,7 @@ struct TileTables
     uint64_t *btab = nullptr;
     void *chunks = nullptr;
     void *strs = nullptr;
+    fpt::ShStr *sh = nullptr; // shared-leader string records (expectation tables)
 };
 
 TileGeom tile_geom(int dtype, size_t n_qubits, size_t n_states);

@@ -246,6 +246,105 @@ uint32_t hoist_bits(fpt::TileChunk const *ch, uint32_t n_chunks, uint32_t d)
     return bits;
 }
 
+// Shared leaders for an expectation group.  Local coordinates may use any basis
+// of V (the coset structure depends on V only): with generators g_0 .. g_m, the
+// strings' x masks (above the low k bits) that first raise the span, the basis
+// g_0, g_1 ^ g_0, ..., g_m ^ g_0 (padded with vectors of the former basis) gives
+// every string whose x is the sum of an odd number of generators local bit k
+// set.  When that holds for every string of the group (at most kMaxShared of
+// them), the group switches to this basis and returns true: the rows with local
+// bit k clear are then leaders for all of its strings.
+bool shared_basis(HostGroup &g, fp_hip_ops const *ops, uint32_t k, uint32_t dh)
+{
+    if (g.mem.empty() || g.mem.size() > size_t(fpt::kMaxShared) || dh == 0)
+        return false;
+    uint64_t const lowmask = (uint64_t(1) << k) - 1;
+    // elimination over the new basis, tracking which basis vectors make up each row
+    std::vector<uint64_t> nb, red, comb;
+    std::vector<uint32_t> rpiv;
+    auto reduce = [&](uint64_t v, uint64_t &c) {
+        c = 0;
+        for (size_t i = 0; i < red.size(); ++i)
+            if ((v >> rpiv[i]) & 1u)
+            {
+                v ^= red[i];
+                c ^= comb[i];
+            }
+        return v;
+    };
+    auto push = [&](uint64_t v) { // v independent of nb
+        uint64_t c = 0;
+        uint64_t const r = reduce(v, c);
+        nb.push_back(v);
+        red.push_back(r);
+        comb.push_back(c ^ (uint64_t(1) << (nb.size() - 1)));
+        rpiv.push_back(63u - uint32_t(__builtin_clzll(r)));
+    };
+    uint64_t g0 = 0;
+    for (LocalStr const &m : g.mem)
+    {
+        uint64_t const xh = ops->x[m.s] & ~lowmask;
+        uint64_t c = 0;
+        if (!xh || !reduce(xh, c))
+            continue;
+        if (nb.empty())
+        {
+            g0 = xh;
+            push(xh);
+        }
+        else
+            push(xh ^ g0);
+    }
+    if (nb.empty())
+        return false;
+    for (uint64_t b : g.b) // pad with the former basis
+    {
+        uint64_t c = 0;
+        if (nb.size() < dh && reduce(b, c))
+            push(b);
+    }
+    if (nb.size() != dh)
+        return false;
+    // coordinates of every string in the new basis; bit 0 must be set
+    std::vector<uint32_t> alpha(g.mem.size());
+    for (size_t i = 0; i < g.mem.size(); ++i)
+    {
+        uint64_t c = 0;
+        uint64_t const rest = reduce(ops->x[g.mem[i].s] & ~lowmask, c);
+        if (rest != 0 || !(c & 1u))
+            return false;
+        alpha[i] = uint32_t(c);
+    }
+    g.b = nb;
+    for (size_t i = 0; i < g.mem.size(); ++i)
+    {
+        LocalStr &m = g.mem[i];
+        uint64_t const z = ops->z[m.s];
+        uint32_t zeta = 0;
+        for (size_t j = 0; j < nb.size(); ++j)
+            zeta |= parity(nb[j] & z) << j;
+        m.A = (alpha[i] << k) | uint32_t(ops->x[m.s] & lowmask);
+        m.Z = (zeta << k) | uint32_t(z & lowmask);
+    }
+    std::stable_sort(g.mem.begin(), g.mem.end(), [](LocalStr const &a, LocalStr const &b) { return a.A < b.A; });
+    return true;
+}
+
+// TileGroup::shared of a shared-leader group of dimension d: the shared form
+// needs full 64 or 128 KiB tiles (2^(d-1) leader rows = 4 passes per wave of
+// 2^rr rows) and the leader bit k inside the lane part (k < rr).  It is used
+// for complex128 only: complex64 tiles already evaluate at about their load
+// time in the set-up-once form (the shared form measured no faster there).
+uint32_t shared_bits(uint32_t d, uint32_t k)
+{
+    uint32_t bits = 0;
+    for (uint32_t W : {8u, 16u})
+        for (uint32_t rr = 0; rr <= 6; ++rr)
+            if (k < rr && d >= rr + 1 && (uint64_t(1) << (d - 1 - rr)) == 4u * W)
+                bits |= 1u << fpt::shared_bit(W, false, rr);
+    return bits;
+}
+
 fpt::TileGroup group_header(HostGroup const &hg, std::vector<uint64_t> &btab)
 {
     fpt::TileGroup g{};
@@ -280,11 +379,25 @@ fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, 
     std::vector<fpt::TileGroup> groups;
     std::vector<uint64_t> btab;
     std::vector<unsigned char> chunks, strs; // raw bytes of the kind's chunk / string records
+    std::vector<fpt::ShStr> sh;              // shared-leader records (expectation tables)
     size_t n_chunks = 0;
-    for (auto const &hg : hgs)
+    for (auto &hg : hgs)
     {
+        bool const shared = kind == kTilesOpExpect && shared_basis(hg, ops, k, dh);
         fpt::TileGroup g = group_header(hg, btab);
         g.chunk_first = uint32_t(n_chunks);
+        if (kind == kTilesOpExpect)
+        {
+            g.shared = shared ? shared_bits(k + dh, k) : 0u;
+            g.str_first = uint32_t(sh.size());
+            g.n_str = uint32_t(hg.mem.size());
+            for (LocalStr const &m : hg.mem)
+            {
+                double const kap = kappa_of(ops->x[m.s], ops->ny[m.s]);
+                sh.push_back(fpt::ShStr{m.zc, m.A, m.Z, ops->ny[m.s] & 1u, 0u, ops->coeffs[2 * m.s] * kap,
+                                        ops->coeffs[2 * m.s + 1] * kap});
+            }
+        }
         if (kind == kTilesOpApply)
         {
             // one record per string, in the operator's order; n_chunks counts them
@@ -394,13 +507,9 @@ fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, 
                                  k + dh);
         groups.push_back(g);
     }
-    uint32_t max_chunks = 0, hoist_all = ~0u, hoist_any = 0;
+    uint32_t max_chunks = 0;
     for (auto const &g : groups)
-    {
         max_chunks = std::max(max_chunks, g.n_chunks);
-        hoist_all &= g.hoist;
-        hoist_any |= g.hoist;
-    }
 
     // reuse an invalidated entry (its host image and device memory) when there is one
     TileTables *t = nullptr;
@@ -427,18 +536,19 @@ fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, 
     t->dh = dh;
     t->n_groups = uint32_t(groups.size());
     t->max_chunks = max_chunks;
-    t->hoist_all = groups.empty() ? 0u : hoist_all;
-    t->hoist_any = hoist_any;
     auto up = [](size_t v) { return (v + 255) / 256 * 256; };
     t->o_btab = up(groups.size() * sizeof(fpt::TileGroup));
     t->o_chunks = t->o_btab + up(btab.size() * sizeof(uint64_t));
     t->o_strs = t->o_chunks + up(chunks.size());
-    size_t const total = t->o_strs + up(strs.size());
+    t->o_sh = t->o_strs + up(strs.size());
+    size_t const total = t->o_sh + up(sh.size() * sizeof(fpt::ShStr));
     t->h.assign(total, 0);
     std::memcpy(t->h.data(), groups.data(), groups.size() * sizeof(fpt::TileGroup));
     std::memcpy(t->h.data() + t->o_btab, btab.data(), btab.size() * sizeof(uint64_t));
     std::memcpy(t->h.data() + t->o_chunks, chunks.data(), chunks.size());
     std::memcpy(t->h.data() + t->o_strs, strs.data(), strs.size());
+    if (!sh.empty())
+        std::memcpy(t->h.data() + t->o_sh, sh.data(), sh.size() * sizeof(fpt::ShStr));
     t->on_device = false;
     t->valid = true;
     *out = t;
@@ -479,6 +589,7 @@ fp_status tile_device(fp_hip_ops const *ops, TileTables *t, hipStream_t stream)
     t->btab = reinterpret_cast<uint64_t *>(base + t->o_btab);
     t->chunks = base + t->o_chunks;
     t->strs = base + t->o_strs;
+    t->sh = reinterpret_cast<fpt::ShStr *>(base + t->o_sh);
     t->on_device = true;
     return FP_OK;
 }
@@ -529,14 +640,21 @@ fp_status choose_op_expect_tiled(fp_hip_ops const *ops, int dtype, size_t n_stat
     return FP_OK;
 }
 
-// TileGroup::hoist bit that the HOIST instance of k_op_expect_tiled checks: it
-// runs 4 passes per step with 2^r leader rows per pass, r = 6 - lsb, or r + 1
-// for complex64 (two states per lane, which needs lsb >= 1).
-uint32_t ev_hoist_bit(int dtype, TileGeom const &geo, bool *can_hoist)
+// Instance of k_op_expect_tiled that runs group g at this launch shape (2 shared
+// leaders, 1 set up once, 0 general), by the kernel's own rule: modes 1 and 2
+// run 2^r leader rows per pass, r = 6 - lsb, or r + 1 for complex64 (two states
+// per lane, which needs lsb >= 1).
+int ev_group_mode(fpt::TileGroup const &g, int dtype, TileGeom const &geo)
 {
     bool const c64 = dtype != FP_C128;
-    *can_hoist = !c64 || geo.lsb >= 1;
-    return fpt::hoist_bit(geo.tile_kb == 128 ? 16u : 8u, 4u, (c64 ? 7u : 6u) - geo.lsb);
+    uint32_t const W = geo.tile_kb == 128 ? 16u : 8u;
+    uint32_t const rrx = (c64 ? 7u : 6u) - geo.lsb;
+    bool const can_pair = !c64 || geo.lsb >= 1;
+    if (can_pair && ((g.shared >> fpt::shared_bit(W, c64, rrx)) & 1u) != 0)
+        return 2;
+    if (can_pair && ((g.hoist >> fpt::hoist_bit(W, 4u, rrx)) & 1u) != 0)
+        return 1;
+    return 0;
 }
 
 fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim,
@@ -583,10 +701,15 @@ fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *stat
         auto set4 = [&](auto T_tag, auto tkb_tag) {
             typedef decltype(T_tag) T;
             constexpr int TKB = decltype(tkb_tag)::value;
-            set(reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<T, false, TKB, false>), TKB << 10);
-            set(reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<T, true, TKB, false>), TKB << 10);
-            set(reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<T, false, TKB, true>), TKB << 10);
-            set(reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<T, true, TKB, true>), TKB << 10);
+            set(reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<T, false, TKB, 0>), TKB << 10);
+            set(reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<T, true, TKB, 0>), TKB << 10);
+            set(reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<T, false, TKB, 1>), TKB << 10);
+            set(reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<T, true, TKB, 1>), TKB << 10);
+            if constexpr (sizeof(T) == 8)
+            {
+                set(reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<T, false, TKB, 2>), TKB << 10);
+                set(reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<T, true, TKB, 2>), TKB << 10);
+            }
         };
         set4(double{}, std::integral_constant<int, 64>{});
         set4(float{}, std::integral_constant<int, 64>{});
@@ -602,26 +725,31 @@ fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *stat
     // Groups whose waves each own at most one fast-path unit run in the HOIST
     // instance, the others in the general one; blocks of the other kind exit at
     // once, so a launch with both kinds runs both instances over the same grid.
-    bool can_hoist = false;
-    uint32_t const hb = ev_hoist_bit(dtype, geo, &can_hoist);
-    bool const run_hoist = can_hoist && ((tt->hoist_any >> hb) & 1u) != 0;
-    bool const run_general = !can_hoist || ((tt->hoist_all >> hb) & 1u) == 0;
+    bool run_mode[3] = {false, false, false};
+    auto const *hgroups = reinterpret_cast<fpt::TileGroup const *>(tt->h.data());
+    for (uint32_t g = 0; g < tt->n_groups; ++g)
+        run_mode[ev_group_mode(hgroups[g], dtype, geo)] = true;
     auto launch = [&](auto T_tag, auto wide_tag, auto tkb_tag, void *typed_out_v) -> fp_status {
         typedef decltype(T_tag) T;
         typedef typename fpk::VecOf<T, 1>::type V;
         constexpr bool WIDE = decltype(wide_tag)::value;
         constexpr int TKB = decltype(tkb_tag)::value;
         auto const *typed_states = static_cast<V const *>(states);
-        for (int pass = 0; pass < 2; ++pass)
+        // the shared-leader instance exists for complex128 only
+        constexpr int kTop = sizeof(T) == 8 ? 2 : 1;
+        for (int mode = 2; mode >= 0; --mode)
         {
-            if (pass == 0 ? !run_hoist : !run_general)
+            if (!run_mode[mode])
                 continue;
-            auto const kfn = pass == 0 ? &fpt::k_op_expect_tiled<T, WIDE, TKB, true>
-                                       : &fpt::k_op_expect_tiled<T, WIDE, TKB, false>;
+            if (mode > kTop)
+                return fail(FP_ERR_RUNTIME, "no shared-leader instance for this precision");
+            auto const kfn = mode == 2   ? &fpt::k_op_expect_tiled<T, WIDE, TKB, kTop>
+                             : mode == 1 ? &fpt::k_op_expect_tiled<T, WIDE, TKB, 1>
+                                         : &fpt::k_op_expect_tiled<T, WIDE, TKB, 0>;
             hipLaunchKernelGGL(kfn, grid, dim3(TKB * 8), lds, st, typed_states, uint64_t(n_states), sshift, geo.lsb,
                                geo.k, geo.d, ranges, uint32_t(n_parts), chunks_y, n_cosets, tt->groups, tt->btab,
                                static_cast<fpt::TileChunk const *>(tt->chunks),
-                               static_cast<fpt::TileStr const *>(tt->strs), partial);
+                               static_cast<fpt::TileStr const *>(tt->strs), tt->sh, partial);
             FP_LAUNCHED();
         }
         hipLaunchKernelGGL(fpk::k_reduce_partials<T>, dim3(uint32_t(n_states)), dim3(fpk::kBlock), 0, st, partial,
@@ -877,10 +1005,12 @@ extern "C" fp_status fp_hip_op_expectation_plan(fp_hip_ops const *ops, int dtype
     plan->tile_rows_log2 = geo.d;
     plan->states_per_block = 1u << geo.lsb;
     plan->hbm_bytes = state_bytes * tt->n_groups;
-    bool can_hoist = false;
-    uint32_t const hb = fprt::ev_hoist_bit(dtype, geo, &can_hoist);
     auto const *groups = reinterpret_cast<fpt::TileGroup const *>(tt->h.data());
-    for (uint32_t g = 0; can_hoist && g < tt->n_groups; ++g)
-        plan->hoisted_groups += (groups[g].hoist >> hb) & 1u;
+    for (uint32_t g = 0; g < tt->n_groups; ++g)
+    {
+        int const mode = fprt::ev_group_mode(groups[g], dtype, geo);
+        plan->hoisted_groups += mode == 1 ? 1u : 0u;
+        plan->shared_groups += mode == 2 ? 1u : 0u;
+    }
     return FP_OK;
 }

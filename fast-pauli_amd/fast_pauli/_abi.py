@@ -101,10 +101,10 @@ class EvPlan(ctypes.Structure):
     """fp_hip_ev_plan"""
     _fields_ = [("kernel", ctypes.c_int), ("n_groups", ctypes.c_uint32), ("tile_rows_log2", ctypes.c_uint32),
                 ("states_per_block", ctypes.c_uint32), ("hbm_bytes", ctypes.c_uint64),
-                ("hoisted_groups", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("hoisted_groups", ctypes.c_uint32), ("shared_groups", ctypes.c_uint32)]
 
     def as_dict(self) -> dict:
-        return {f: int(getattr(self, f)) for f, _ in self._fields_ if f != "reserved"}
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
 
 
 def op_expectation_plan(ops, dtype: int, dim: int, n_states: int) -> dict:

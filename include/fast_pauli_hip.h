@@ -172,8 +172,10 @@ fp_status fp_hip_op_expectation(fp_hip_ops const *ops, int dtype, void const *st
  *     of dimension tile_rows_log2 (with its low bits); each coset of it is a tile of
  *     2^tile_rows_log2 rows x states_per_block states, loaded once into LDS per group.
  * hbm_bytes: state bytes the chosen kernel reads per call, (strings or groups) x 2^n x S x B.
- * hoisted_groups: tiled groups whose waves each own at most one single-string unit, set up once
- *   per block (the faster instance of the tiled kernel); the others set units up per tile. */
+ * shared_groups: tiled groups whose strings all pair rows with one common local bit clear, so
+ *   that one LDS read of a leader row serves every string (the fastest instance).
+ * hoisted_groups: other tiled groups whose waves each own at most one single-string unit, set
+ *   up once per block; the remaining groups set units up per tile. */
 enum
 {
     FP_EV_STREAM = 0,
@@ -187,7 +189,7 @@ typedef struct fp_hip_ev_plan
     uint32_t states_per_block;
     uint64_t hbm_bytes;
     uint32_t hoisted_groups;
-    uint32_t reserved;
+    uint32_t shared_groups;
 } fp_hip_ev_plan;
 fp_status fp_hip_op_expectation_plan(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_states,
                                      fp_hip_ev_plan *plan);

@@ -271,10 +271,10 @@ def test_op_expectation_plan(hip, monkeypatch):
     assert p["tile_rows_log2"] == 8 and p["states_per_block"] == 16
     assert N // 8 <= p["n_groups"] <= N // 4
     assert p["hbm_bytes"] == p["n_groups"] * (1 << n) * S * 16
-    assert N // 16 <= p["hoisted_groups"] <= p["n_groups"]
+    assert N // 16 <= p["shared_groups"] and p["shared_groups"] + p["hoisted_groups"] <= p["n_groups"]
     p64 = _abi.op_expectation_plan(ops, C64, 1 << n, S)
     assert p64["tile_rows_log2"] == 9 and p64["n_groups"] <= p["n_groups"]
-    assert p64["hoisted_groups"] <= p64["n_groups"]
+    assert p64["shared_groups"] == 0 and 0 < p64["hoisted_groups"] <= p64["n_groups"]
     single = _abi.Ops(syn.random_codes(1, n, seed=5))
     assert _abi.op_expectation_plan(single, C128, 1 << n, S)["kernel"] == _abi.FP_EV_STREAM
     monkeypatch.setenv("FAST_PAULI_EV_KERNEL", "stream")
@@ -287,11 +287,13 @@ def test_op_expectation_plan(hip, monkeypatch):
 @pytest.mark.parametrize("dtype", [C128, C64])
 @pytest.mark.parametrize("S", [1, 2, 16, 24])
 def test_tiled_expectation_both_instances(hip, monkeypatch, dtype, S):
-    """Groups whose waves each own one single-string unit run in the set-up-once
-    instance of the tiled kernel (two states per lane for complex64, which needs
-    2 states or more per block), the others in the per-tile instance; one launch
-    holding both kinds of group equals the oracle, and so do launches of each
-    kind alone."""
+    """The tiled kernel's instances: complex128 groups whose strings all share
+    leader rows (at most 8 strings, rebased so that every local x mask has the
+    lowest span bit) run in the shared-leader instance, other groups whose waves
+    each own one single-string unit in the set-up-once instance (two states per
+    lane for complex64, which needs 2 states or more per block), the rest in the
+    per-tile instance; launches mixing them equal the oracle, and so do launches
+    of one kind alone."""
     H.say(f"tiled expectation instances dtype={dtype} S={S}")
     monkeypatch.setenv("FAST_PAULI_EV_KERNEL", "tiled")
     n = 16
@@ -305,15 +307,19 @@ def test_tiled_expectation_both_instances(hip, monkeypatch, dtype, S):
         ops = _abi.Ops(codes, coeffs)
         p = _abi.op_expectation_plan(ops, dtype, 1 << n, S)
         assert p["kernel"] == _abi.FP_EV_TILED
-        h = p["hoisted_groups"]
-        # a group is hoisted when it has 1, 2, 4 or 8 chunks of one string each (a
-        # random group may take more than 8 strings when x masks fall in its span)
+        h = p["hoisted_groups"] + p["shared_groups"]
+        assert h <= p["n_groups"], (name, p)
+        # a random group of up to 8 strings shares leaders; one that took more than
+        # 8 strings (x masks falling in its span) may still set up once when it has
+        # at most 8 chunks of one string each
         if dtype == C64 and S == 1:
             assert h == 0, name  # one state per block: no two-state lanes
-        elif name == "random":
-            assert 0 < h <= p["n_groups"], (name, p)
+        elif name == "random" and dtype == C128:
+            assert p["shared_groups"] > 0, (name, p)
+        elif name == "random":  # the shared-leader instance is complex128 only
+            assert p["shared_groups"] == 0 and p["hoisted_groups"] > 0, (name, p)
         elif name == "shared_x":
-            assert h == 0, (name, p)  # strings sharing an x mask share chunks
+            assert h == 0, (name, p)  # 40 strings of 4 x masks: one group, chunks of 4
         else:
             assert 0 < h < p["n_groups"], (name, p)
         got = H.op_expectation(ops, psi, dtype)
