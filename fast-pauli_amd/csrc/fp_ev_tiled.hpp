@@ -747,16 +747,19 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
         // per string, 4 bits each: case (Im flag * 4 + ci) in bits 0..2 of a
         // nibble of `cases`, and zi in the matching nibble of `zis`
         uint32_t cases = 0, zis = 0;
+        uint32_t zcs[kMaxShared]; // coset sign masks (coset indices < 2^32: host condition)
         Acc acc_s[kMaxShared][NS];
 #pragma unroll
         for (int s = 0; s < kMaxShared; ++s)
         {
             pb[s] = la0;
+            zcs[s] = 0;
 #pragma unroll
             for (int n = 0; n < NS; ++n)
                 acc_s[s][n] = Acc(0);
             if (s < int(n_sh))
             {
+                zcs[s] = uint32_t(ss[s].zc);
                 uint32_t const A = ss[s].A, Z = ss[s].Z;
                 pb[s] = ((((af ^ (A & lmask)) + (wpart ^ (A & wmask))) << lg.lsbx) | lg.tx) * ESZ;
                 neg_bits |= uint32_t(__popc((af + wpart) & Z) & 1) << s;
@@ -767,12 +770,11 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
         cases = __builtin_amdgcn_readfirstlane(cases);
         zis = __builtin_amdgcn_readfirstlane(zis);
         auto compute = [&](uint64_t q) {
-            // coset signs of every string, from the scalar cache before the LDS reads
+            // coset signs of every string
             uint32_t cneg = 0;
 #pragma unroll
             for (int s = 0; s < kMaxShared; ++s)
-                if (s < int(n_sh))
-                    cneg |= uint32_t(__popcll(q & ss[s].zc) & 1) << s;
+                cneg |= uint32_t(__popc(uint32_t(q) & zcs[s]) & 1) << s;
             LV L[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i)

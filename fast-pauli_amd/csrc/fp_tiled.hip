@@ -388,7 +388,8 @@ fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, 
         g.chunk_first = uint32_t(n_chunks);
         if (kind == kTilesOpExpect)
         {
-            g.shared = shared ? shared_bits(k + dh, k) : 0u;
+            // the shared-leader instance keeps coset indices in 32 bits
+            g.shared = shared && ops->n_qubits - (k + dh) <= 32 ? shared_bits(k + dh, k) : 0u;
             g.str_first = uint32_t(sh.size());
             g.n_str = uint32_t(hg.mem.size());
             for (LocalStr const &m : hg.mem)
