@@ -75,7 +75,7 @@ struct TileGroup
 
 // Strings of a shared-leader group: every local x mask has bit k set, so the
 // leader rows (local bit k clear) are common to all of them.
-constexpr int kMaxShared = 8;
+constexpr int kMaxShared = 9;
 
 struct ShStr
 {
@@ -744,8 +744,7 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
         uint32_t const wmask = ~((1u << (rr + 3u)) - 1u);
         uint32_t pb[kMaxShared];
         uint32_t neg_bits = 0; // bit s: parity((af + wpart) & Z_s)
-        // per string, 4 bits each: case (Im flag * 4 + ci) in bits 0..2 of a
-        // nibble of `cases`, and zi in the matching nibble of `zis`
+        // per string: case (Im flag * 4 + ci) in 3 bits of `cases`, zi in 2 bits of `zis`
         uint32_t cases = 0, zis = 0;
         uint32_t zcs[kMaxShared]; // coset sign masks (coset indices < 2^32: host condition)
         Acc acc_s[kMaxShared][NS];
@@ -763,8 +762,8 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
                 uint32_t const A = ss[s].A, Z = ss[s].Z;
                 pb[s] = ((((af ^ (A & lmask)) + (wpart ^ (A & wmask))) << lg.lsbx) | lg.tx) * ESZ;
                 neg_bits |= uint32_t(__popc((af + wpart) & Z) & 1) << s;
-                cases |= (((ss[s].imag ? 4u : 0u) + ((A >> (rr + 1u)) & 3u)) << (4 * s));
-                zis |= ((Z >> (rr + 1u)) & 3u) << (4 * s);
+                cases |= (((ss[s].imag ? 4u : 0u) + ((A >> (rr + 1u)) & 3u)) << (3 * s));
+                zis |= ((Z >> (rr + 1u)) & 3u) << (2 * s);
             }
         }
         cases = __builtin_amdgcn_readfirstlane(cases);
@@ -784,8 +783,8 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
             {
                 if (s < int(n_sh))
                 {
-                    uint32_t const cse = (cases >> (4 * s)) & 7u;
-                    uint32_t const zi = (zis >> (4 * s)) & 3u;
+                    uint32_t const cse = (cases >> (3 * s)) & 7u;
+                    uint32_t const zi = (zis >> (2 * s)) & 3u;
                     Acc const s1 = (zi & 1u) ? Acc(-1) : Acc(1);
                     Acc const s2 = (zi & 2u) ? Acc(-1) : Acc(1);
                     Acc const cs = ((cneg >> s) & 1u) ? Acc(-1) : Acc(1);

@@ -80,7 +80,8 @@ struct TileTables
     fpt::ShStr *sh = nullptr; // shared-leader string records (expectation tables)
 };
 
-TileGeom tile_geom(int dtype, size_t n_qubits, size_t n_states);
+// `expect`: tiles for the expectation kernel (complex128 takes 128-byte row slices), else apply
+TileGeom tile_geom(int dtype, size_t n_qubits, size_t n_states, bool expect);
 fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, TileTables **out);
 fp_status tile_device(fp_hip_ops const *ops, TileTables *t, hipStream_t stream);
 void invalidate_tiles(fp_hip_ops *ops);

@@ -83,10 +83,17 @@ struct Span
 namespace fprt
 {
 
-TileGeom tile_geom(int dtype, size_t n_qubits, size_t n_states)
+TileGeom tile_geom(int dtype, size_t n_qubits, size_t n_states, bool expect)
 {
     uint32_t const esz = dtype == FP_C128 ? 16u : 8u;
-    uint32_t const row_target = env_u32("FAST_PAULI_TILE_ROW_BYTES", 256u);
+    // complex128 expectation tiles with a power-of-two state count take 128-byte
+    // row slices (8 states): 2^9-row tiles, so groups of 9 strings in the
+    // shared-leader form, against 8 strings with 256-byte rows; the slices of a
+    // row are fetched by blocks of one XCD.  (With other state counts the slices
+    // would not align with rows.)
+    bool const pow2 = n_states != 0 && (n_states & (n_states - 1)) == 0;
+    uint32_t const row_target =
+        env_u32("FAST_PAULI_TILE_ROW_BYTES", expect && dtype == FP_C128 && pow2 ? 128u : 256u);
     uint32_t const tile_kb = env_u32("FAST_PAULI_TILE_KB", 64u) >= 128u ? 128u : 64u;
     uint32_t const elems = tile_kb * 1024u / esz;
     uint32_t sblk = 1;
@@ -632,7 +639,7 @@ fp_status choose_op_expect_tiled(fp_hip_ops const *ops, int dtype, size_t n_stat
     bool const forced = v && std::strcmp(v, "tiled") == 0;
     if (!forced && ops->n_strings < 2)
         return FP_OK;
-    TileGeom const geo = tile_geom(dtype, ops->n_qubits, n_states);
+    TileGeom const geo = tile_geom(dtype, ops->n_qubits, n_states, true);
     TileTables *tt = nullptr;
     fp_status const st = tile_tables(ops, kTilesOpExpect, geo.k, geo.dh, &tt);
     if (st != FP_OK)
@@ -661,7 +668,7 @@ int ev_group_mode(fpt::TileGroup const &g, int dtype, TileGeom const &geo)
 fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim,
                               size_t n_states, int accumulate, hipStream_t st)
 {
-    TileGeom const geo = tile_geom(dtype, ops->n_qubits, n_states);
+    TileGeom const geo = tile_geom(dtype, ops->n_qubits, n_states, true);
     TileTables *tt = nullptr;
     fp_status s = tile_tables(ops, kTilesOpExpect, geo.k, geo.dh, &tt);
     if (s != FP_OK)
@@ -789,7 +796,7 @@ fp_status choose_op_apply_tiled(fp_hip_ops const *ops, int dtype, size_t n_state
     size_t const state_bytes = (size_t(1) << ops->n_qubits) * n_states * (dtype == FP_C128 ? 16 : 8);
     if (!forced && (ops->n_strings < 8 || state_bytes < (size_t(64) << 20)))
         return FP_OK;
-    TileGeom const geo = tile_geom(dtype, ops->n_qubits, n_states);
+    TileGeom const geo = tile_geom(dtype, ops->n_qubits, n_states, false);
     TileTables *tt = nullptr;
     fp_status const st = tile_tables(ops, kTilesOpApply, geo.k, geo.dh, &tt);
     if (st != FP_OK)
@@ -801,7 +808,7 @@ fp_status choose_op_apply_tiled(fp_hip_ops const *ops, int dtype, size_t n_state
 fp_status run_op_apply_tiled(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim,
                              size_t n_states, int accumulate, hipStream_t st)
 {
-    TileGeom const geo = tile_geom(dtype, ops->n_qubits, n_states);
+    TileGeom const geo = tile_geom(dtype, ops->n_qubits, n_states, false);
     TileTables *tt = nullptr;
     fp_status s = tile_tables(ops, kTilesOpApply, geo.k, geo.dh, &tt);
     if (s != FP_OK)
@@ -997,7 +1004,7 @@ extern "C" fp_status fp_hip_op_expectation_plan(fp_hip_ops const *ops, int dtype
         plan->hbm_bytes = state_bytes * ops->n_strings;
         return FP_OK;
     }
-    fprt::TileGeom const geo = fprt::tile_geom(dtype, ops->n_qubits, n_states);
+    fprt::TileGeom const geo = fprt::tile_geom(dtype, ops->n_qubits, n_states, true);
     fprt::TileTables *tt = nullptr;
     if ((st = fprt::tile_tables(ops, fprt::kTilesOpExpect, geo.k, geo.dh, &tt)) != FP_OK)
         return st;

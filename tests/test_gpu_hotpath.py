@@ -260,20 +260,22 @@ def test_op_expectation_kernels(hip, monkeypatch, kernel, dtype, kind, n, N, S):
 
 
 def test_op_expectation_plan(hip, monkeypatch):
-    """fp_hip_op_expectation_plan: random strings group 8 per tile at 16
-    complex128 states (256-byte rows, 2^8-row tiles); the bytes follow the
-    groups; a forced streaming kernel reports one state read per string."""
+    """fp_hip_op_expectation_plan: at 16 complex128 states, tiles take 128-byte
+    row slices (8 states, 2^9-row tiles) and random strings group about 9 per
+    tile; complex64 takes 16 states per block (2^9 rows, 8 strings per group);
+    the bytes follow the groups; a forced streaming kernel reports one state
+    read per string."""
     H.say("op_expectation plan")
     n, N, S = 16, 256, 16
     ops = _abi.Ops(syn.random_codes(N, n, seed=4), syn.random_coeffs(N, seed=4))
     p = _abi.op_expectation_plan(ops, C128, 1 << n, S)
     assert p["kernel"] == _abi.FP_EV_TILED
-    assert p["tile_rows_log2"] == 8 and p["states_per_block"] == 16
-    assert N // 8 <= p["n_groups"] <= N // 4
+    assert p["tile_rows_log2"] == 9 and p["states_per_block"] == 8
+    assert N // 9 <= p["n_groups"] <= N // 4
     assert p["hbm_bytes"] == p["n_groups"] * (1 << n) * S * 16
-    assert N // 16 <= p["shared_groups"] and p["shared_groups"] + p["hoisted_groups"] <= p["n_groups"]
+    assert N // 18 <= p["shared_groups"] and p["shared_groups"] + p["hoisted_groups"] <= p["n_groups"]
     p64 = _abi.op_expectation_plan(ops, C64, 1 << n, S)
-    assert p64["tile_rows_log2"] == 9 and p64["n_groups"] <= p["n_groups"]
+    assert p64["tile_rows_log2"] == 9 and p64["states_per_block"] == 16 and N // 8 <= p64["n_groups"] <= N // 4
     assert p64["shared_groups"] == 0 and 0 < p64["hoisted_groups"] <= p64["n_groups"]
     single = _abi.Ops(syn.random_codes(1, n, seed=5))
     assert _abi.op_expectation_plan(single, C128, 1 << n, S)["kernel"] == _abi.FP_EV_STREAM
