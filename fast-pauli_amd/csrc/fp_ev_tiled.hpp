@@ -740,9 +740,10 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
         uint32_t const af = insert_zero_bit32(lg.pf, k);
         uint32_t const wpart = (wave * 4u) << (rr + 1u);
         uint32_t const la0 = (((af + wpart) << lg.lsbx) | lg.tx) * ESZ;
-        uint32_t const lmask = (1u << (rr + 1u)) - 1u;
-        uint32_t const wmask = ~((1u << (rr + 3u)) - 1u);
-        uint32_t pb[kMaxShared];
+        // partner base of string s: row (af | wpart) ^ (A without its pass bits), so
+        // byte address la0 ^ px[s] (ESZ a power of two, tx below the row bits)
+        uint32_t const passmask = 3u << (rr + 1u);
+        uint32_t px[kMaxShared];
         uint32_t neg_bits = 0; // bit s: parity((af + wpart) & Z_s)
         // per string: case (Im flag * 4 + ci) in 3 bits of `cases`, zi in 2 bits of `zis`
         uint32_t cases = 0, zis = 0;
@@ -751,7 +752,7 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
 #pragma unroll
         for (int s = 0; s < kMaxShared; ++s)
         {
-            pb[s] = la0;
+            px[s] = 0;
             zcs[s] = 0;
 #pragma unroll
             for (int n = 0; n < NS; ++n)
@@ -760,7 +761,7 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
             {
                 zcs[s] = uint32_t(ss[s].zc);
                 uint32_t const A = ss[s].A, Z = ss[s].Z;
-                pb[s] = ((((af ^ (A & lmask)) + (wpart ^ (A & wmask))) << lg.lsbx) | lg.tx) * ESZ;
+                px[s] = ((A & ~passmask) << lg.lsbx) * ESZ;
                 neg_bits |= uint32_t(__popc((af + wpart) & Z) & 1) << s;
                 cases |= (((ss[s].imag ? 4u : 0u) + ((A >> (rr + 1u)) & 3u)) << (3 * s));
                 zis |= ((Z >> (rr + 1u)) & 3u) << (2 * s);
@@ -778,53 +779,61 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
 #pragma unroll
             for (int i = 0; i < 4; ++i)
                 L[i] = *reinterpret_cast<LV const *>(smem + (la0 + uint32_t(i) * STRIDE));
+            // partner values of string s
+            auto load_R = [&](int s, LV(&R)[4]) {
+                uint32_t const base = la0 ^ px[s];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    R[j] = *reinterpret_cast<LV const *>(smem + (base + uint32_t(j) * STRIDE));
+            };
+            auto eval = [&](int s, LV const(&R)[4]) {
+                uint32_t const cse = (cases >> (3 * s)) & 7u;
+                uint32_t const zi = (zis >> (2 * s)) & 3u;
+                Acc const s1 = (zi & 1u) ? Acc(-1) : Acc(1);
+                Acc const s2 = (zi & 2u) ? Acc(-1) : Acc(1);
+                Acc const cs = ((cneg >> s) & 1u) ? Acc(-1) : Acc(1);
+                // sum_i (-1)^{parity(i & zi)} G(L[i], R[i ^ ci]) per state
+                auto body = [&](auto im_tag, auto ci_tag) {
+                    constexpr bool IM = decltype(im_tag)::value;
+                    constexpr int CI = decltype(ci_tag)::value;
+#pragma unroll
+                    for (int n = 0; n < NS; ++n)
+                    {
+                        Acc G[4];
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+                        {
+                            Acc const lr = Acc(L[i][2 * n]), li = Acc(L[i][2 * n + 1]);
+                            Acc const pr = Acc(R[i ^ CI][2 * n]), pi = Acc(R[i ^ CI][2 * n + 1]);
+                            G[i] = IM ? fma(lr, pi, -(li * pr)) : fma(lr, pr, li * pi);
+                        }
+                        Acc const t0 = fma(s1, G[1], G[0]);
+                        Acc const t1 = fma(s1, G[3], G[2]);
+                        acc_s[s][n] = fma(cs, fma(s2, t1, t0), acc_s[s][n]);
+                    }
+                };
+                typedef std::true_type Im;
+                typedef std::false_type Re;
+                switch (cse)
+                {
+                case 0: body(Re{}, std::integral_constant<int, 0>{}); break;
+                case 1: body(Re{}, std::integral_constant<int, 1>{}); break;
+                case 2: body(Re{}, std::integral_constant<int, 2>{}); break;
+                case 3: body(Re{}, std::integral_constant<int, 3>{}); break;
+                case 4: body(Im{}, std::integral_constant<int, 0>{}); break;
+                case 5: body(Im{}, std::integral_constant<int, 1>{}); break;
+                case 6: body(Im{}, std::integral_constant<int, 2>{}); break;
+                default: body(Im{}, std::integral_constant<int, 3>{}); break;
+                }
+            };
 #pragma unroll
             for (int s = 0; s < kMaxShared; ++s)
             {
                 if (s < int(n_sh))
                 {
-                    uint32_t const cse = (cases >> (3 * s)) & 7u;
-                    uint32_t const zi = (zis >> (2 * s)) & 3u;
-                    Acc const s1 = (zi & 1u) ? Acc(-1) : Acc(1);
-                    Acc const s2 = (zi & 2u) ? Acc(-1) : Acc(1);
-                    Acc const cs = ((cneg >> s) & 1u) ? Acc(-1) : Acc(1);
                     LV R[4];
-#pragma unroll
-                    for (int j = 0; j < 4; ++j)
-                        R[j] = *reinterpret_cast<LV const *>(smem + (pb[s] + uint32_t(j) * STRIDE));
-                    // sum_i (-1)^{parity(i & zi)} G(L[i], R[i ^ ci]) per state
-                    auto body = [&](auto im_tag, auto ci_tag) {
-                        constexpr bool IM = decltype(im_tag)::value;
-                        constexpr int CI = decltype(ci_tag)::value;
-#pragma unroll
-                        for (int n = 0; n < NS; ++n)
-                        {
-                            Acc G[4];
-#pragma unroll
-                            for (int i = 0; i < 4; ++i)
-                            {
-                                Acc const lr = Acc(L[i][2 * n]), li = Acc(L[i][2 * n + 1]);
-                                Acc const pr = Acc(R[i ^ CI][2 * n]), pi = Acc(R[i ^ CI][2 * n + 1]);
-                                G[i] = IM ? fma(lr, pi, -(li * pr)) : fma(lr, pr, li * pi);
-                            }
-                            Acc const t0 = fma(s1, G[1], G[0]);
-                            Acc const t1 = fma(s1, G[3], G[2]);
-                            acc_s[s][n] = fma(cs, fma(s2, t1, t0), acc_s[s][n]);
-                        }
-                    };
-                    typedef std::true_type Im;
-                    typedef std::false_type Re;
-                    switch (cse)
-                    {
-                    case 0: body(Re{}, std::integral_constant<int, 0>{}); break;
-                    case 1: body(Re{}, std::integral_constant<int, 1>{}); break;
-                    case 2: body(Re{}, std::integral_constant<int, 2>{}); break;
-                    case 3: body(Re{}, std::integral_constant<int, 3>{}); break;
-                    case 4: body(Im{}, std::integral_constant<int, 0>{}); break;
-                    case 5: body(Im{}, std::integral_constant<int, 1>{}); break;
-                    case 6: body(Im{}, std::integral_constant<int, 2>{}); break;
-                    default: body(Im{}, std::integral_constant<int, 3>{}); break;
-                    }
+                    load_R(s, R);
+                    eval(s, R);
                 }
             }
         };
