@@ -18,6 +18,7 @@
 #include <string>
 #include <vector>
 
+#include "fp_dense.hpp"
 #include "fp_kernels.hpp"
 #include "fp_mfma.hpp"
 #include "fp_runtime.hpp"
@@ -1467,6 +1468,153 @@ fp_status fp_hip_gemm(int contraction, size_t m, size_t n, size_t k, void const 
     if (st != FP_OK)
         return fail(st, "%s", fpm::last_error());
     return FP_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Dense and sparse materialisation (debugging, interop)
+// ---------------------------------------------------------------------------
+
+namespace
+{
+
+fp_status check_dense(fp_hip_ops const *ops, int dtype, void const *out, size_t dim, size_t n_mats)
+{
+    fp_status st = check_ops(ops);
+    if (st != FP_OK)
+        return st;
+    if (dtype != FP_C128 && dtype != FP_C64)
+        return fail(FP_ERR_INVALID_ARGUMENT, "unknown dtype %d", dtype);
+    if (ops->n_strings == 0 || ops->n_qubits == 0)
+        return fail(FP_ERR_INVALID_ARGUMENT, "operator has no Pauli strings");
+    if (ops->n_qubits > 30)
+        return fail(FP_ERR_INVALID_ARGUMENT, "a dense matrix of %zu qubits is not addressable", ops->n_qubits);
+    size_t const want = size_t(1) << ops->n_qubits;
+    if (dim != want)
+        return fail(FP_ERR_INVALID_ARGUMENT, "tensor dimension (%zu) must match the dimension of the operators (%zu)",
+                    dim, want);
+    size_t bytes = 0;
+    if (__builtin_mul_overflow(dim * dim, n_mats, &bytes) ||
+        __builtin_mul_overflow(bytes, size_t(dtype == FP_C128 ? 16 : 8), &bytes))
+        return fail(FP_ERR_INVALID_ARGUMENT, "dense tensor size overflows");
+    if (!out)
+        return fail(FP_ERR_INVALID_ARGUMENT, "NULL data pointer");
+    uintptr_t const align = dtype == FP_C128 ? 16 : 8;
+    if ((reinterpret_cast<uintptr_t>(out) & (align - 1)) != 0)
+        return fail(FP_ERR_INVALID_ARGUMENT, "device buffers must be aligned to the complex element size (%zu bytes)",
+                    size_t(align));
+    return FP_OK;
+}
+
+uint32_t grid_for(uint64_t n)
+{
+    return uint32_t(std::max<uint64_t>(1, std::min<uint64_t>((n + fpk::kBlock - 1) / fpk::kBlock, 65536)));
+}
+
+// k_to_tensor over n_mats (K) matrices, after clearing them unless accumulating
+fp_status run_to_tensor(fp_hip_ops const *ops, int dtype, void *out, size_t dim, size_t n_mats, bool assign,
+                        size_t s0, void const *coef, uint32_t cstride, int folded, bool clear, hipStream_t s)
+{
+    uint64_t const n_rows = uint64_t(n_mats) * dim;
+    uint32_t const n = uint32_t(ops->n_qubits), N = uint32_t(ops->n_strings);
+    if (clear)
+    {
+        uint64_t const n_el = n_rows * dim;
+        if (dtype == FP_C128)
+            hipLaunchKernelGGL(fpd::k_fill_zero<double>, dim3(grid_for(n_el)), dim3(fpk::kBlock), 0, s,
+                               static_cast<fpk::VecOf<double, 1>::type *>(out), n_el);
+        else
+            hipLaunchKernelGGL(fpd::k_fill_zero<float>, dim3(grid_for(n_el)), dim3(fpk::kBlock), 0, s,
+                               static_cast<fpk::VecOf<float, 1>::type *>(out), n_el);
+        FP_LAUNCHED();
+    }
+    auto launch = [&](auto T_tag) {
+        typedef decltype(T_tag) T;
+        auto *o = static_cast<typename fpk::VecOf<T, 1>::type *>(out);
+        auto const *c = static_cast<T const *>(coef);
+        if (assign)
+            hipLaunchKernelGGL((fpd::k_to_tensor<T, true>), dim3(grid_for(n_rows)), dim3(fpk::kBlock), 0, s, o, n,
+                               n_rows, ops->d_x, ops->d_z, ops->d_ny, uint32_t(s0), N, c, cstride, folded);
+        else
+            hipLaunchKernelGGL((fpd::k_to_tensor<T, false>), dim3(grid_for(n_rows)), dim3(fpk::kBlock), 0, s, o, n,
+                               n_rows, ops->d_x, ops->d_z, ops->d_ny, uint32_t(s0), N, c, cstride, folded);
+    };
+    if (dtype == FP_C128)
+        launch(double{});
+    else
+        launch(float{});
+    FP_LAUNCHED();
+    return FP_OK;
+}
+
+} // namespace
+
+extern "C"
+{
+
+fp_status fp_hip_string_sparse_repr(fp_hip_ops const *ops, size_t string_index, int dtype, uint64_t *cols,
+                                    void *vals, size_t dim, void *stream)
+{
+    fp_status st = check_shape(ops, dtype, cols, vals, dim, 1);
+    if (st != FP_OK)
+        return st;
+    if (string_index >= ops->n_strings)
+        return fail(FP_ERR_INVALID_ARGUMENT, "string index %zu out of range (%zu strings)", string_index,
+                    ops->n_strings);
+    if ((reinterpret_cast<uintptr_t>(cols) & 7u) != 0)
+        return fail(FP_ERR_INVALID_ARGUMENT, "cols must be aligned to 8 bytes");
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    uint64_t const x = ops->x[string_index], z = ops->z[string_index];
+    uint32_t const ny = ops->ny[string_index];
+    if (dtype == FP_C128)
+        hipLaunchKernelGGL(fpd::k_sparse_repr<double>, dim3(grid_for(dim)), dim3(fpk::kBlock), 0, s, cols,
+                           static_cast<fpk::VecOf<double, 1>::type *>(vals), uint64_t(dim), x, z, ny);
+    else
+        hipLaunchKernelGGL(fpd::k_sparse_repr<float>, dim3(grid_for(dim)), dim3(fpk::kBlock), 0, s, cols,
+                           static_cast<fpk::VecOf<float, 1>::type *>(vals), uint64_t(dim), x, z, ny);
+    FP_LAUNCHED();
+    return FP_OK;
+}
+
+fp_status fp_hip_string_to_tensor(fp_hip_ops const *ops, size_t string_index, int dtype, void *out, size_t dim,
+                                  void *stream)
+{
+    fp_status st = check_dense(ops, dtype, out, dim, 1);
+    if (st != FP_OK)
+        return st;
+    if (string_index >= ops->n_strings)
+        return fail(FP_ERR_INVALID_ARGUMENT, "string index %zu out of range (%zu strings)", string_index,
+                    ops->n_strings);
+    return run_to_tensor(ops, dtype, out, dim, 1, true, string_index, nullptr, 0, 0, false,
+                         static_cast<hipStream_t>(stream));
+}
+
+fp_status fp_hip_op_to_tensor(fp_hip_ops const *ops, int dtype, void *out, size_t dim, int accumulate, void *stream)
+{
+    fp_status st = check_dense(ops, dtype, out, dim, 1);
+    if (st != FP_OK)
+        return st;
+    void const *coef = dtype == FP_C128 ? static_cast<void const *>(ops->d_cf128) : static_cast<void const *>(ops->d_cf64);
+    return run_to_tensor(ops, dtype, out, dim, 1, false, 0, coef, 1, 1, accumulate == 0,
+                         static_cast<hipStream_t>(stream));
+}
+
+fp_status fp_hip_summed_to_tensor(fp_hip_ops const *ops, int dtype, void *out, size_t dim, int accumulate,
+                                  void *stream)
+{
+    fp_status st = check_ops(ops);
+    if (st != FP_OK)
+        return st;
+    if (ops->n_operators == 0)
+        return fail(FP_ERR_INVALID_ARGUMENT, "SummedPauliOp has no coefficient matrix");
+    if ((st = check_dense(ops, dtype, out, dim, ops->n_operators)) != FP_OK)
+        return st;
+    if (ops->n_operators > 0xffffffffu)
+        return fail(FP_ERR_INVALID_ARGUMENT, "too many operators");
+    void const *coef = dtype == FP_C128 ? static_cast<void const *>(ops->d_h128) : static_cast<void const *>(ops->d_h64);
+    return run_to_tensor(ops, dtype, out, dim, ops->n_operators, false, 0, coef, uint32_t(ops->n_operators), 0,
+                         accumulate == 0, static_cast<hipStream_t>(stream));
 }
 
 } // extern "C"

@@ -218,6 +218,30 @@ fp_status fp_hip_summed_apply(fp_hip_ops const *ops, int dtype, void const *stat
 fp_status fp_hip_summed_apply_weighted(fp_hip_ops const *ops, int dtype, void const *states, void const *data,
                                        void *out, size_t dim, size_t n_states, int accumulate, void *stream);
 
+/* ---- dense and sparse materialisation on the device (debugging, interop) --------
+ * Values are IEEE-equal to the reference's: m(j) = (-i)^{nY} (-1)^{popcount(j & z)} formed
+ * as the reference forms it, c * m as std::complex multiplies, strings added in order. */
+
+/* get_sparse_repr (fast_pauli/cpp/include/__pauli_string.hpp:49-118) of string `string_index`:
+ * cols[j] = j ^ x (dim uint64 entries), vals[j] = m(j) (dim complex values of `dtype`). */
+fp_status fp_hip_string_sparse_repr(fp_hip_ops const *ops, size_t string_index, int dtype, uint64_t *cols,
+                                    void *vals, size_t dim, void *stream);
+
+/* PauliString::to_tensor (fast_pauli/cpp/include/__pauli_string.hpp:549-560): out(j, j ^ x) = m(j)
+ * for every row j of the (dim x dim) row-major matrix `out`; other elements are left unchanged. */
+fp_status fp_hip_string_to_tensor(fp_hip_ops const *ops, size_t string_index, int dtype, void *out, size_t dim,
+                                  void *stream);
+
+/* PauliOp::to_tensor (fast_pauli/cpp/include/__pauli_op.hpp:558-571): out (dim x dim) (+)=
+ * sum_s c_s P_s, strings in order (accumulate = 0 clears out first). */
+fp_status fp_hip_op_to_tensor(fp_hip_ops const *ops, int dtype, void *out, size_t dim, int accumulate,
+                              void *stream);
+
+/* SummedPauliOp::to_tensor (fast_pauli/cpp/include/__summed_pauli_op.hpp:644-665): out
+ * (n_operators x dim x dim) (+)= sum_s h(s,k) P_s for each operator k, strings in order. */
+fp_status fp_hip_summed_to_tensor(fp_hip_ops const *ops, int dtype, void *out, size_t dim, int accumulate,
+                                  void *stream);
+
 /* ---- uneven batches: many operators in one persistent, graph-captured launch ----
  * Entries may differ in qubit count, string count, state count, dtype and kind.
  * Outputs of one entry must not overlap any other entry's buffers.  The first
