@@ -632,11 +632,16 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
                 return;
             }
         }
+        // MODE 2 swizzles rows: local row a is stored at a ^ ((a >> 1) & 1), so that the two
+        // rows of a 16-lane LDS access group (rows differing in bit 0 or in bit 1 only) always
+        // fall on different halves of the 64 banks when a row slice is 128 bytes.  Bit 1 of
+        // the row of element tid + m * kThreads is a bit of tid (kThreads >= 2^(lsb + 2)).
+        uint32_t const swz = MODE == 2 ? ((tid >> (lsb + 1u)) & 1u) << lsb : 0u;
         if (full)
         {
 #pragma unroll
             for (uint32_t m = 0; m < EPT; ++m)
-                tile[tid + m * kThreads] = rg[m];
+                tile[(tid + m * kThreads) ^ swz] = rg[m];
         }
         else
         {
@@ -645,7 +650,7 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
             {
                 uint32_t const e = tid + m * kThreads;
                 if (e < tile_elems)
-                    tile[e] = rg[m];
+                    tile[e ^ swz] = rg[m];
             }
         }
     };
@@ -739,7 +744,10 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
         ShStr const *ss = sh + grp.str_first;
         uint32_t const af = insert_zero_bit32(lg.pf, k);
         uint32_t const wpart = (wave * 4u) << (rr + 1u);
-        uint32_t const la0 = (((af + wpart) << lg.lsbx) | lg.tx) * ESZ;
+        // rows are stored swizzled (see store_tile): row a at a ^ ((a >> 1) & 1), a map that
+        // is linear over XOR and touches only row bits 0 and 1, below the pass bits
+        auto swz_row = [](uint32_t a) { return a ^ ((a >> 1) & 1u); };
+        uint32_t const la0 = ((swz_row(af + wpart) << lg.lsbx) | lg.tx) * ESZ;
         // partner base of string s: row (af | wpart) ^ (A without its pass bits), so
         // byte address la0 ^ px[s] (ESZ a power of two, tx below the row bits)
         uint32_t const passmask = 3u << (rr + 1u);
@@ -761,7 +769,7 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
             {
                 zcs[s] = uint32_t(ss[s].zc);
                 uint32_t const A = ss[s].A, Z = ss[s].Z;
-                px[s] = ((A & ~passmask) << lg.lsbx) * ESZ;
+                px[s] = (swz_row(A & ~passmask) << lg.lsbx) * ESZ;
                 neg_bits |= uint32_t(__popc((af + wpart) & Z) & 1) << s;
                 cases |= (((ss[s].imag ? 4u : 0u) + ((A >> (rr + 1u)) & 3u)) << (3 * s));
                 zis |= ((Z >> (rr + 1u)) & 3u) << (2 * s);
