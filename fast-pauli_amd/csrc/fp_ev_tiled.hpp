@@ -74,8 +74,14 @@ struct TileGroup
 };
 
 // Strings of a shared-leader group: every local x mask has bit k set, so the
-// leader rows (local bit k clear) are common to all of them.
-constexpr int kMaxShared = 9;
+// leader rows (local bit k clear) are common to all of them.  At most
+// max_shared(W) strings for the instance with W waves per block (its per-string
+// registers), kMaxShared over all instances.
+constexpr int kMaxShared = 10;
+__host__ __device__ constexpr int max_shared(uint32_t W)
+{
+    return W == 16 ? 10 : 9;
+}
 
 struct ShStr
 {
@@ -636,21 +642,21 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
         // rows of a 16-lane LDS access group (rows differing in bit 0 or in bit 1 only) always
         // fall on different halves of the 64 banks when a row slice is 128 bytes.  Bit 1 of
         // the row of element tid + m * kThreads is a bit of tid (kThreads >= 2^(lsb + 2)).
-        uint32_t const swz = MODE == 2 ? ((tid >> (lsb + 1u)) & 1u) << lsb : 0u;
+        // (tid + m * kThreads) ^ swz = (tid ^ swz) + m * kThreads: one base, immediate offsets
+        uint32_t const tsw = MODE == 2 ? tid ^ (((tid >> (lsb + 1u)) & 1u) << lsb) : tid;
         if (full)
         {
 #pragma unroll
             for (uint32_t m = 0; m < EPT; ++m)
-                tile[(tid + m * kThreads) ^ swz] = rg[m];
+                tile[tsw + m * kThreads] = rg[m];
         }
         else
         {
 #pragma unroll
             for (uint32_t m = 0; m < EPT; ++m)
             {
-                uint32_t const e = tid + m * kThreads;
-                if (e < tile_elems)
-                    tile[e ^ swz] = rg[m];
+                if (tid + m * kThreads < tile_elems)
+                    tile[tsw + m * kThreads] = rg[m];
             }
         }
     };
@@ -740,7 +746,8 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
         uint32_t const sb2 = PAIRED ? (sblk >> 1) : sblk;
         Lanes const lg = PAIRED ? Lanes{lsb - 1u, lane & (sb2 - 1u), r + 1u, lane >> (lsb - 1u)} : plain;
         uint32_t const rr = lg.rr;
-        uint32_t const n_sh = grp.n_str; // <= kMaxShared
+        constexpr int KMS = max_shared(W);
+        uint32_t const n_sh = grp.n_str; // <= KMS (host condition)
         ShStr const *ss = sh + grp.str_first;
         uint32_t const af = insert_zero_bit32(lg.pf, k);
         uint32_t const wpart = (wave * 4u) << (rr + 1u);
@@ -751,14 +758,14 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
         // partner base of string s: row (af | wpart) ^ (A without its pass bits), so
         // byte address la0 ^ px[s] (ESZ a power of two, tx below the row bits)
         uint32_t const passmask = 3u << (rr + 1u);
-        uint32_t px[kMaxShared];
+        uint32_t px[KMS];
         uint32_t neg_bits = 0; // bit s: parity((af + wpart) & Z_s)
         // per string: case (Im flag * 4 + ci) in 3 bits of `cases`, zi in 2 bits of `zis`
         uint32_t cases = 0, zis = 0;
-        uint32_t zcs[kMaxShared]; // coset sign masks (coset indices < 2^32: host condition)
-        Acc acc_s[kMaxShared][NS];
+        uint32_t zcs[KMS]; // coset sign masks (coset indices < 2^32: host condition)
+        Acc acc_s[KMS][NS];
 #pragma unroll
-        for (int s = 0; s < kMaxShared; ++s)
+        for (int s = 0; s < KMS; ++s)
         {
             px[s] = 0;
             zcs[s] = 0;
@@ -781,7 +788,7 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
             // coset signs of every string
             uint32_t cneg = 0;
 #pragma unroll
-            for (int s = 0; s < kMaxShared; ++s)
+            for (int s = 0; s < KMS; ++s)
                 cneg |= uint32_t(__popc(uint32_t(q) & zcs[s]) & 1) << s;
             LV L[4];
 #pragma unroll
@@ -835,7 +842,7 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
                 }
             };
 #pragma unroll
-            for (int s = 0; s < kMaxShared; ++s)
+            for (int s = 0; s < KMS; ++s)
             {
                 if (s < int(n_sh))
                 {
@@ -848,7 +855,7 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
         tile_loop(compute);
         // per-lane sign and string scale, once per block
 #pragma unroll
-        for (int s = 0; s < kMaxShared; ++s)
+        for (int s = 0; s < KMS; ++s)
         {
             if (s < int(n_sh))
             {

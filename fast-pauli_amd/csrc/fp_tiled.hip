@@ -94,7 +94,9 @@ TileGeom tile_geom(int dtype, size_t n_qubits, size_t n_states, bool expect)
     bool const pow2 = n_states != 0 && (n_states & (n_states - 1)) == 0;
     uint32_t const row_target =
         env_u32("FAST_PAULI_TILE_ROW_BYTES", expect && dtype == FP_C128 && pow2 ? 128u : 256u);
-    uint32_t const tile_kb = env_u32("FAST_PAULI_TILE_KB", 64u) >= 128u ? 128u : 64u;
+    // 128 KiB tiles (FAST_PAULI_TILE_KB=128) exist for the expectation kernel only; the
+    // tiled apply kernel's tiles are 64 KiB
+    uint32_t const tile_kb = expect && env_u32("FAST_PAULI_TILE_KB", 64u) >= 128u ? 128u : 64u;
     uint32_t const elems = tile_kb * 1024u / esz;
     uint32_t sblk = 1;
     while (sblk < n_states && sblk * esz < row_target)
@@ -339,15 +341,17 @@ bool shared_basis(HostGroup &g, fp_hip_ops const *ops, uint32_t k, uint32_t dh)
 
 // TileGroup::shared of a shared-leader group of dimension d: the shared form
 // needs full 64 or 128 KiB tiles (2^(d-1) leader rows = 4 passes per wave of
-// 2^rr rows) and the leader bit k inside the lane part (k < rr).  It is used
+// 2^rr rows), the leader bit k inside the lane part (k < rr) and at most
+// max_shared(W) strings (the instance's per-string registers).  It is used
 // for complex128 only: complex64 tiles already evaluate at about their load
 // time in the set-up-once form (the shared form measured no faster there).
-uint32_t shared_bits(uint32_t d, uint32_t k)
+uint32_t shared_bits(uint32_t d, uint32_t k, size_t n_str)
 {
     uint32_t bits = 0;
     for (uint32_t W : {8u, 16u})
         for (uint32_t rr = 0; rr <= 6; ++rr)
-            if (k < rr && d >= rr + 1 && (uint64_t(1) << (d - 1 - rr)) == 4u * W)
+            if (k < rr && d >= rr + 1 && (uint64_t(1) << (d - 1 - rr)) == 4u * W &&
+                n_str <= size_t(fpt::max_shared(W)))
                 bits |= 1u << fpt::shared_bit(W, false, rr);
     return bits;
 }
@@ -396,7 +400,7 @@ fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, 
         if (kind == kTilesOpExpect)
         {
             // the shared-leader instance keeps coset indices in 32 bits
-            g.shared = shared && ops->n_qubits - (k + dh) <= 32 ? shared_bits(k + dh, k) : 0u;
+            g.shared = shared && ops->n_qubits - (k + dh) <= 32 ? shared_bits(k + dh, k, hg.mem.size()) : 0u;
             g.str_first = uint32_t(sh.size());
             g.n_str = uint32_t(hg.mem.size());
             for (LocalStr const &m : hg.mem)

@@ -109,7 +109,7 @@ def test_op_apply_bit_exact(hip, dtype, n, N, S):
     assert np.array_equal(got, exp)
 
 
-@pytest.mark.parametrize("kernel", ["stream", "tiled", "tiled_wide"])
+@pytest.mark.parametrize("kernel", ["stream", "tiled", "tiled_wide", "tiled_kb128"])
 @pytest.mark.parametrize("dtype", [C128, C64])
 @pytest.mark.parametrize("kind,n,N,S", [
     ("random", 2, 5, 1), ("random", 8, 40, 3), ("random", 11, 64, 16), ("random", 13, 200, 7),
@@ -124,6 +124,9 @@ def test_op_apply_kernels(hip, monkeypatch, kernel, dtype, kind, n, N, S):
     monkeypatch.setenv("FAST_PAULI_APPLY_KERNEL", kernel.split("_")[0])
     if kernel == "tiled_wide":
         monkeypatch.setenv("FAST_PAULI_TILE_WIDE", "1")
+    if kernel == "tiled_kb128":
+        # the expectation kernel's tile-size knob leaves the apply tiles at 64 KiB
+        monkeypatch.setenv("FAST_PAULI_TILE_KB", "128")
     cdt = H.CDT[dtype]
     codes = _structured_codes(kind, N, n, seed=7 * n + N)
     strings = syn.codes_to_strings(codes)
@@ -225,7 +228,7 @@ def _structured_codes(kind, N, n, seed):
     raise ValueError(kind)
 
 
-@pytest.mark.parametrize("kernel", ["stream", "tiled", "tiled_wide", "auto"])
+@pytest.mark.parametrize("kernel", ["stream", "tiled", "tiled_wide", "tiled_kb128", "auto"])
 @pytest.mark.parametrize("dtype", [C128, C64])
 @pytest.mark.parametrize("kind,n,N,S", [
     ("random", 2, 5, 1), ("random", 8, 40, 3), ("random", 11, 64, 16), ("random", 13, 200, 7),
@@ -241,6 +244,8 @@ def test_op_expectation_kernels(hip, monkeypatch, kernel, dtype, kind, n, N, S):
         monkeypatch.setenv("FAST_PAULI_EV_KERNEL", kernel.split("_")[0])
     if kernel == "tiled_wide":  # 64-bit element offsets, normally used from 2^32 elements up
         monkeypatch.setenv("FAST_PAULI_TILE_WIDE", "1")
+    if kernel == "tiled_kb128":  # 128 KiB tiles: 1024-thread blocks, shared groups of up to 10
+        monkeypatch.setenv("FAST_PAULI_TILE_KB", "128")
     cdt = H.CDT[dtype]
     codes = _structured_codes(kind, N, n, seed=5 * n + N)
     strings = syn.codes_to_strings(codes)
