@@ -93,8 +93,8 @@ def main() -> int:
     if args.pmc_gemm:
         return pmc_gemm()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is "
-                          "design-only (see DESIGN.md)", "n_gpus": args.gpus}), flush=True)
+        print(json.dumps({"error": "this benchmark runs on one GPU only; sharding over several GPUs "
+                          "is not implemented", "n_gpus": args.gpus}), flush=True)
         return 2
     from fast_pauli import _abi
     from fast_pauli import synthetic as syn
