@@ -294,6 +294,7 @@ def main() -> int:
     if not args.no_extra:
         result["workloads"] = {
             "pauli_op_apply_n10": bench_op_apply_n10(args),
+            "pauli_op_apply_n24": bench_op_apply_n24(args),
             "summed_expectation_n14": bench_summed_n14(args),
             "uneven_batch_256": bench_uneven_batch(args),
             "serving": bench_serving(args),
@@ -372,6 +373,31 @@ def bench_op_apply_n10(args) -> dict:
         out["cpu_threads"] = O.max_threads()
         out["bit_exact_vs_oracle_seq"] = bool(np.array_equal(got, ref))
     return out
+
+
+def bench_op_apply_n24(args) -> dict:
+    """PauliOp::apply, n=24, 64 strings, 16 states (HBM-resident): the plan's bytes against time."""
+    from fast_pauli import _abi, synthetic as syn
+
+    n, N, S = 24, 64, 16
+    log("workload: PauliOp.apply n=24, 64 strings, 16 states")
+    codes = syn.random_codes(N, n, seed=args.seed + 1)
+    coeffs = syn.random_coeffs(N, seed=args.seed + 1)
+    ops = _abi.Ops(codes, coeffs)
+    res = {"config": "n=24, 64 strings (uniform IXYZ), 16 normalised states, overwrite"}
+    for dt, name, esz in ((_abi.FP_C128, "c128", 16), (_abi.FP_C64, "c64", 8)):
+        d_in = _abi.DeviceBuffer((1 << n) * S * esz)
+        d_out = _abi.DeviceBuffer((1 << n) * S * esz)
+        _abi.call("fp_hip_fill_states", dt, d_in.ptr, 1 << n, S, args.seed, 1, None)
+        plan = _abi.op_apply_plan(ops, dt, 1 << n, S)
+        ms = _time_ms(lambda: _abi.call("fp_hip_op_apply", ops.handle, dt, d_in.ptr, d_out.ptr, 1 << n, S, 0, None),
+                      3)
+        gbs = plan["hbm_bytes"] / (ms / 1e3) / 1e9
+        res[name] = {"ms": ms, "strings_x_states_per_s": N * S / (ms / 1e3), "plan": plan,
+                     "achieved_gbs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS,
+                     "streaming_model_bytes": (N + 1) * (1 << n) * S * esz}
+        del d_in, d_out
+    return res
 
 
 def bench_summed_n14(args) -> dict:

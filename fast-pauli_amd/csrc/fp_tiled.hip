@@ -1026,3 +1026,40 @@ extern "C" fp_status fp_hip_op_expectation_plan(fp_hip_ops const *ops, int dtype
     }
     return FP_OK;
 }
+
+extern "C" fp_status fp_hip_op_apply_plan(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_states, int accumulate,
+                                          fp_hip_ev_plan *plan)
+{
+    if (!ops || !plan)
+        return fail(FP_ERR_INVALID_ARGUMENT, "NULL ops or plan");
+    if (dtype != FP_C128 && dtype != FP_C64)
+        return fail(FP_ERR_INVALID_ARGUMENT, "unknown dtype %d", dtype);
+    if (n_states == 0 || dim != (size_t(1) << ops->n_qubits))
+        return fail(FP_ERR_INVALID_ARGUMENT, "shape (%zu, %zu) does not match %zu qubits", dim, n_states,
+                    ops->n_qubits);
+    bool tiled = false;
+    fp_status st = fprt::choose_op_apply_tiled(ops, dtype, n_states, &tiled);
+    if (st != FP_OK)
+        return st;
+    uint64_t const state_bytes = uint64_t(dim) * n_states * (dtype == FP_C128 ? 16u : 8u);
+    uint64_t const out_reads = accumulate ? 1u : 0u;
+    *plan = fp_hip_ev_plan{};
+    if (!tiled)
+    {
+        plan->kernel = FP_EV_STREAM;
+        plan->n_groups = uint32_t(ops->n_strings);
+        plan->hbm_bytes = state_bytes * (ops->n_strings + 1u + out_reads);
+        return FP_OK;
+    }
+    fprt::TileGeom const geo = fprt::tile_geom(dtype, ops->n_qubits, n_states, false);
+    fprt::TileTables *tt = nullptr;
+    if ((st = fprt::tile_tables(ops, fprt::kTilesOpApply, geo.k, geo.dh, &tt)) != FP_OK)
+        return st;
+    uint64_t const G = tt->n_groups;
+    plan->kernel = FP_EV_TILED;
+    plan->n_groups = uint32_t(G);
+    plan->tile_rows_log2 = geo.d;
+    plan->states_per_block = 1u << geo.lsb;
+    plan->hbm_bytes = state_bytes * (3u * G - 1u + out_reads);
+    return FP_OK;
+}

@@ -194,6 +194,15 @@ typedef struct fp_hip_ev_plan
 fp_status fp_hip_op_expectation_plan(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_states,
                                      fp_hip_ev_plan *plan);
 
+/* Which kernel fp_hip_op_apply runs for these shapes, in the same record (no reference
+ * counterpart: for measurement and triage).  kernel 0 (FP_EV_STREAM): every string reads the
+ * state once and the output is written once.  kernel 1 (FP_EV_TILED): groups of consecutive
+ * strings, one launch each, every group reading the state and reading and writing the output
+ * (the first group does not read it unless accumulating).  hbm_bytes counts those reads and
+ * writes; hoisted_groups and shared_groups are 0. */
+fp_status fp_hip_op_apply_plan(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_states, int accumulate,
+                               fp_hip_ev_plan *plan);
+
 /* Per-string expectation values without coefficients, as a REAL (n_strings x n_states) matrix
  * ev(j,t) = <psi_t|P_j|psi_t> (FP64 for FP_C128, FP32 for FP_C64).  Pauli strings are Hermitian,
  * so the imaginary part is exactly zero.  Stage 1 of SummedPauliOp::expectation_value

@@ -152,9 +152,17 @@ def test_op_apply_tiled_is_default_for_large_states(hip, monkeypatch, dtype):
     ops = _abi.Ops(syn.random_codes(N, n, seed=41), syn.random_coeffs(N, seed=41))
     psi = syn.random_states(1 << n, S, seed=41).astype(cdt)
     auto = H.op_apply(ops, psi, dtype)
+    B = psi.nbytes
+    p = _abi.op_apply_plan(ops, dtype, 1 << n, S)
+    assert p["kernel"] == _abi.FP_EV_TILED and 0 < p["n_groups"] < N
+    G = p["n_groups"]
+    assert p["hbm_bytes"] == (3 * G - 1) * B  # state + out per group; the first group overwrites
+    assert _abi.op_apply_plan(ops, dtype, 1 << n, S, accumulate=True)["hbm_bytes"] == 3 * G * B
     monkeypatch.setenv("FAST_PAULI_APPLY_KERNEL", "stream")
     ref = H.op_apply(ops, psi, dtype)
     assert np.array_equal(auto, ref)
+    p = _abi.op_apply_plan(ops, dtype, 1 << n, S)
+    assert p["kernel"] == _abi.FP_EV_STREAM and p["hbm_bytes"] == (N + 1) * B
 
 
 def test_op_apply_known_answers(hip):
