@@ -86,14 +86,14 @@ namespace fprt
 TileGeom tile_geom(int dtype, size_t n_qubits, size_t n_states, bool expect)
 {
     uint32_t const esz = dtype == FP_C128 ? 16u : 8u;
-    // complex128 expectation tiles with a power-of-two state count take 128-byte
-    // row slices (8 states): 2^9-row tiles, so groups of 9 strings in the
-    // shared-leader form, against 8 strings with 256-byte rows; the slices of a
-    // row are fetched by blocks of one XCD.  (With other state counts the slices
-    // would not align with rows.)
+    // complex128 tiles with a power-of-two state count take 128-byte row slices
+    // (8 states): 2^9-row tiles, so groups of 9 strings (in the shared-leader form
+    // for expectation values), against 8 strings with 256-byte rows; the slices of
+    // a row are fetched by blocks of one XCD.  Apply gains the same way (256 random
+    // strings: 29 groups instead of 32).  With other state counts the slices would
+    // not align with rows.
     bool const pow2 = n_states != 0 && (n_states & (n_states - 1)) == 0;
-    uint32_t const row_target =
-        env_u32("FAST_PAULI_TILE_ROW_BYTES", expect && dtype == FP_C128 && pow2 ? 128u : 256u);
+    uint32_t const row_target = env_u32("FAST_PAULI_TILE_ROW_BYTES", dtype == FP_C128 && pow2 ? 128u : 256u);
     // 128 KiB tiles (FAST_PAULI_TILE_KB=128) exist for the expectation kernel only; the
     // tiled apply kernel's tiles are 64 KiB
     uint32_t const tile_kb = expect && env_u32("FAST_PAULI_TILE_KB", 64u) >= 128u ? 128u : 64u;
