@@ -19,8 +19,10 @@
 // A thread owns "slots" of the tile: one element, or (PAIRED: complex64 full
 // tiles with a power-of-two state count, S_blk >= 2) the two neighbouring
 // states of one row, moved by 16-byte loads, stores and LDS reads.  Coset bases
-// advance by one deposited addition per tile, and a slot's output values are
-// requested before the tile store, so their latency overlaps it.
+// advance by one deposited addition per tile.  complex128 (32-bit offsets)
+// requests the next coset's output values together with its state values, in a
+// second register set; other forms request a coset's output values just before
+// its tile store, so their latency overlaps the store.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -148,18 +150,23 @@ __global__ __launch_bounds__(kThreads, 4) void k_op_apply_tiled(typename fpk::Ve
     uint32_t const rm = 9u - lsbx; // log2(kThreads) - lsbx
     uint32_t const a_lane = tid >> lsbx;
     uint32_t const lane_mask = (1u << rm) - 1u;
-    for (uint64_t q = blockIdx.x; q < n_cosets; q += q_step)
-    {
-        // this coset's output values, requested before the tile store
-        LV acc[EPT];
+    // this coset's output values (zero for the first group of an overwrite)
+    auto read_out = [&](LV(&acc)[EPT], uint64_t cbq) {
 #pragma unroll
         for (uint32_t m = 0; m < EPT; ++m)
         {
             acc[m] = LV(0);
             if (!init_zero && slot_live(m))
-                acc[m] = out_l[index_of(cb, m)];
+                acc[m] = out_l[index_of(cbq, m)];
         }
-        __syncthreads(); // the previous tile is no longer read
+    };
+    auto write_out = [&](LV const(&acc)[EPT], uint64_t cbq) {
+#pragma unroll
+        for (uint32_t m = 0; m < EPT; ++m)
+            if (slot_live(m))
+                __builtin_nontemporal_store(acc[m], &out_l[index_of(cbq, m)]);
+    };
+    auto store_tile = [&]() {
 #pragma unroll
         for (uint32_t m = 0; m < EPT; ++m)
         {
@@ -167,10 +174,9 @@ __global__ __launch_bounds__(kThreads, 4) void k_op_apply_tiled(typename fpk::Ve
             if (full || e < tile_slots)
                 tile[e] = reg[m];
         }
-        __syncthreads();
-        uint64_t const cbn = ((cb | ~cmask) + cstep) & cmask;
-        if (q + q_step < n_cosets)
-            fetch(cbn);
+    };
+    // the group's strings, in order, on the tile in LDS
+    auto compute = [&](uint64_t q, LV(&acc)[EPT]) {
         for (uint32_t s = 0; s < n_str; ++s)
         {
             ApplyStr const st = gs[s];
@@ -210,11 +216,61 @@ __global__ __launch_bounds__(kThreads, 4) void k_op_apply_tiled(typename fpk::Ve
                 }
             }
         }
-#pragma unroll
-        for (uint32_t m = 0; m < EPT; ++m)
-            if (slot_live(m))
-                __builtin_nontemporal_store(acc[m], &out_l[index_of(cb, m)]);
-        cb = cbn;
+    };
+    // complex128 with 32-bit offsets has the registers to request the next coset's
+    // output values with its state values (two output sets, alternating); other
+    // forms request a coset's output values just before its tile store
+    constexpr bool PF = sizeof(T) == 8 && !WIDE;
+    if constexpr (PF)
+    {
+        LV acc0[EPT], acc1[EPT];
+        if (blockIdx.x < n_cosets)
+            read_out(acc0, cb);
+        for (uint64_t q = blockIdx.x; q < n_cosets; q += 2 * q_step)
+        {
+            __syncthreads(); // the previous tile is no longer read
+            store_tile();
+            __syncthreads();
+            uint64_t const cb1 = ((cb | ~cmask) + cstep) & cmask;
+            if (q + q_step < n_cosets)
+            {
+                fetch(cb1);
+                read_out(acc1, cb1);
+            }
+            compute(q, acc0);
+            write_out(acc0, cb);
+            if (q + q_step >= n_cosets)
+                break;
+            __syncthreads();
+            store_tile();
+            __syncthreads();
+            uint64_t const cb2 = ((cb1 | ~cmask) + cstep) & cmask;
+            if (q + 2 * q_step < n_cosets)
+            {
+                fetch(cb2);
+                read_out(acc0, cb2);
+            }
+            compute(q + q_step, acc1);
+            write_out(acc1, cb1);
+            cb = cb2;
+        }
+    }
+    else
+    {
+        for (uint64_t q = blockIdx.x; q < n_cosets; q += q_step)
+        {
+            LV acc[EPT];
+            read_out(acc, cb); // requested before the tile store
+            __syncthreads();   // the previous tile is no longer read
+            store_tile();
+            __syncthreads();
+            uint64_t const cbn = ((cb | ~cmask) + cstep) & cmask;
+            if (q + q_step < n_cosets)
+                fetch(cbn);
+            compute(q, acc);
+            write_out(acc, cb);
+            cb = cbn;
+        }
     }
 }
 
