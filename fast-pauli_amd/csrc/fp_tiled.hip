@@ -823,9 +823,11 @@ fp_status run_op_apply_tiled(fp_hip_ops const *ops, int dtype, void const *state
     uint32_t const chunks_y = uint32_t((n_states + (size_t(1) << geo.lsb) - 1) >> geo.lsb);
     if (chunks_y > 65535u)
         return fail(FP_ERR_INVALID_ARGUMENT, "too many states for the tiled apply kernel");
-    uint64_t const blocks_x =
-        std::max<uint64_t>(1, std::min<uint64_t>(n_cosets, (env_u32("FAST_PAULI_TILE_BLOCKS", 8192) + chunks_y - 1) /
-                                                               chunks_y));
+    // about four tiles per block (a block with one tile has nothing to overlap its
+    // loads with), between 1024 and 8192 blocks; FAST_PAULI_TILE_BLOCKS overrides
+    uint64_t const tiles = n_cosets * chunks_y;
+    uint64_t const target = env_u32("FAST_PAULI_TILE_BLOCKS", uint32_t(std::clamp<uint64_t>(tiles / 4, 1024, 8192)));
+    uint64_t const blocks_x = std::max<uint64_t>(1, std::min<uint64_t>(n_cosets, (target + chunks_y - 1) / chunks_y));
     uint32_t const sshift = (n_states & (n_states - 1)) == 0 ? log2u(n_states) : 64u;
     bool const wide = (uint64_t(dim) * n_states) >> 32 != 0 || env_u32("FAST_PAULI_TILE_WIDE", 0) == 1;
     // complex64 full tiles with a power-of-two state count that one block's
