@@ -187,6 +187,8 @@ __device__ __forceinline__ void chunk_rows_n(typename fpk::VecOf<T, 1>::type con
             else
                 b[k] = a[k];
         }
+        // all U row blocks' LDS reads in flight before the first use
+        __builtin_amdgcn_sched_barrier(0);
         uint32_t flip[CNT];
 #pragma unroll
         for (int s = 0; s < CNT; ++s)
