@@ -261,6 +261,21 @@ def test_summed_square():
         np.testing.assert_allclose(dsq[k], dense[k] @ dense[k], atol=1e-12)
 
 
+def test_summed_square_random_strings():
+    """square() (products found by bit masks, phases letter by letter) equals A_k @ A_k
+    for random strings with every letter, and the fallback by string keys agrees."""
+    rng = np.random.default_rng(11)
+    n, N, K = 5, 12, 3
+    letters = np.array(list("IXYZ"))
+    strings = sorted({"".join(rng.choice(letters, n)) for _ in range(N)})
+    c = rng.standard_normal((len(strings), K)) + 1j * rng.standard_normal((len(strings), K))
+    op = fp.SummedPauliOp(strings, c)
+    sq = op.square()
+    dense, dsq = op.to_tensor(), sq.to_tensor()
+    for k in range(K):
+        np.testing.assert_allclose(dsq[k], dense[k] @ dense[k], atol=1e-12)
+
+
 def test_summed_pickle_clone():
     strings = ["XI", "IZ", "YY"]
     c = np.arange(6, dtype=np.complex128).reshape(3, 2)
