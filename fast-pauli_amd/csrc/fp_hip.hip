@@ -1490,6 +1490,8 @@ fp_status check_dense(fp_hip_ops const *ops, int dtype, void const *out, size_t 
         return fail(FP_ERR_INVALID_ARGUMENT, "operator has no Pauli strings");
     if (ops->n_qubits > 30)
         return fail(FP_ERR_INVALID_ARGUMENT, "a dense matrix of %zu qubits is not addressable", ops->n_qubits);
+    if (ops->n_strings > 0xffffffffu)
+        return fail(FP_ERR_INVALID_ARGUMENT, "too many strings for the dense form (%zu)", ops->n_strings);
     size_t const want = size_t(1) << ops->n_qubits;
     if (dim != want)
         return fail(FP_ERR_INVALID_ARGUMENT, "tensor dimension (%zu) must match the dimension of the operators (%zu)",
