@@ -165,9 +165,12 @@ def main() -> int:
         call("fp_hip_string_apply_batch", ops_apply.handle, 0, C128, psi.ptr, out.ptr, dim, S, 1.0, 0.0, 0, stream)
         call("fp_hip_op_expectation", ops_ev.handle, C128, psi.ptr, ev.ptr, dim, S, 0, stream)
         stream_ev_once(C128, psi, ev)
+        # PauliOp.apply of the n=24 workload (64 strings): one coset-tiled launch per group
+        ops_op = _op_apply_n24_ops(args)
+        call("fp_hip_op_apply", ops_op.handle, C128, psi.ptr, out.ptr, dim, S, 0, stream)
         _abi.synchronize()
         log(f"pmc pass done: copy 4 GiB, apply_batch, expectation_value x{N} ({ev_kernel[plan128['kernel']]}, "
-            f"then fpk::k_op_expect), n={n}, {S} states, complex128")
+            f"then fpk::k_op_expect), PauliOp.apply x{ops_op.n_strings}, n={n}, {S} states, complex128")
         return 0
 
     # achievable copy bandwidth on this device (read + write of one 4 GiB buffer)
@@ -309,7 +312,8 @@ def main() -> int:
 
 
 def measured_traffic(kernel: str, n: int, S: int, n_strings: int):
-    """HBM bytes per launch of `kernel` from the committed counter summary
+    """HBM bytes per call of `kernel` (summed over its launches when one call makes
+    several, as the tiled PauliOp.apply does per string group) from the committed counter summary
     (profiles/pmc_traffic_n24_s16.json: FETCH_SIZE and WRITE_SIZE, each scaled by
     the plain copy kernel's reading for a known byte count), for the shape it
     was measured at; None otherwise."""
@@ -375,15 +379,20 @@ def bench_op_apply_n10(args) -> dict:
     return out
 
 
+def _op_apply_n24_ops(args):
+    """The operator of the n=24 PauliOp.apply workload: 64 random strings."""
+    from fast_pauli import _abi, synthetic as syn
+
+    return _abi.Ops(syn.random_codes(64, 24, seed=args.seed + 1), syn.random_coeffs(64, seed=args.seed + 1))
+
+
 def bench_op_apply_n24(args) -> dict:
     """PauliOp::apply, n=24, 64 strings, 16 states (HBM-resident): the plan's bytes against time."""
-    from fast_pauli import _abi, synthetic as syn
+    from fast_pauli import _abi
 
     n, N, S = 24, 64, 16
     log("workload: PauliOp.apply n=24, 64 strings, 16 states")
-    codes = syn.random_codes(N, n, seed=args.seed + 1)
-    coeffs = syn.random_coeffs(N, seed=args.seed + 1)
-    ops = _abi.Ops(codes, coeffs)
+    ops = _op_apply_n24_ops(args)
     res = {"config": "n=24, 64 strings (uniform IXYZ), 16 normalised states, overwrite"}
     for dt, name, esz in ((_abi.FP_C128, "c128", 16), (_abi.FP_C64, "c64", 8)):
         d_in = _abi.DeviceBuffer((1 << n) * S * esz)
@@ -396,6 +405,9 @@ def bench_op_apply_n24(args) -> dict:
         res[name] = {"ms": ms, "strings_x_states_per_s": N * S / (ms / 1e3), "plan": plan,
                      "achieved_gbs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS,
                      "streaming_model_bytes": (N + 1) * (1 << n) * S * esz}
+        if dt == _abi.FP_C128:
+            # HBM bytes of all group launches from the committed counter summary
+            res[name]["traffic"] = measured_traffic("fpa::k_op_apply_tiled", n, S, N)
         del d_in, d_out
     return res
 
