@@ -64,6 +64,14 @@ inline bool is_device_pointer(void const *p)
     return d != 0;
 }
 
+// Whether p is device memory, without initialising a device first (false where no
+// device is present): host-only helpers such as to_tensor stay usable there.
+inline bool is_device_view(void const *p)
+{
+    int d = 0;
+    return fp_hip_is_device_pointer(p, &d) == FP_OK && d != 0;
+}
+
 // Owning device allocation.
 class Buffer
 {

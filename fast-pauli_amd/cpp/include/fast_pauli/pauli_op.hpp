@@ -255,11 +255,20 @@ template <std::floating_point T, typename H = std::complex<T>> struct PauliOp
                                   states.extent(1), true);
     }
 
-    // Dense matrix (__pauli_op.hpp:558-571); host side, accumulates.
+    // Dense matrix (__pauli_op.hpp:558-571), accumulated: on the host, or on the
+    // device when the view is device memory.
     template <class M>
         requires MdspanOfRank<M, 2>
     void to_tensor(M output) const
     {
+        if (!pauli_strings.empty() && hip::is_device_view(output.data_handle()))
+        {
+            if (output.extent(0) != dim() || output.extent(1) != dim())
+                throw std::invalid_argument("Output tensor must have the same dimensions as the PauliOp");
+            hip::check(fp_hip_op_to_tensor(device_ops(), hip::dtype_of<T>(), output.data_handle(), dim(), 1, nullptr));
+            hip::check(fp_hip_stream_synchronize(nullptr));
+            return;
+        }
         for (std::size_t i = 0; i < pauli_strings.size(); ++i)
         {
             auto [cols, vals] = get_sparse_repr<T>(pauli_strings[i].paulis);

@@ -250,13 +250,20 @@ struct PauliString
                                       states.extent(1), c, true);
     }
 
-    // Dense matrix (debugging helper, __pauli_string.hpp:549-560); host side.
+    // Dense matrix (debugging helper, __pauli_string.hpp:549-560): on the host, or
+    // on the device when the view is device memory.
     template <std::floating_point T, class M>
         requires MdspanOfRank<M, 2>
     void to_tensor(M output) const
     {
         if (output.extent(0) != dim() || output.extent(1) != dim())
             throw std::invalid_argument("Output tensor must have the same dimensions as the PauliString");
+        if (hip::is_device_view(output.data_handle()))
+        {
+            hip::check(fp_hip_string_to_tensor(device_ops(), 0, hip::dtype_of<T>(), output.data_handle(), dim(), nullptr));
+            hip::check(fp_hip_stream_synchronize(nullptr));
+            return;
+        }
         std::uint64_t x, z;
         detail::masks_of(paulis, x, z);
         std::size_t ny = 0;

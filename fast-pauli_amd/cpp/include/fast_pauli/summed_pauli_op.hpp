@@ -284,10 +284,21 @@ template <std::floating_point T> struct SummedPauliOp
         return SummedPauliOp(sq, Tensor<2>(out.data(), sq.size(), K));
     }
 
+    // Dense matrices (__summed_pauli_op.hpp:644-665), accumulated: on the host, or
+    // on the device when the view is device memory.
     template <class M>
         requires MdspanOfRank<M, 3>
     void to_tensor(M A_k_out) const
     {
+        if (!pauli_strings.empty() && hip::is_device_view(A_k_out.data_handle()))
+        {
+            if (A_k_out.extent(0) != n_operators() || A_k_out.extent(1) != dim() || A_k_out.extent(2) != dim())
+                throw std::invalid_argument("Output tensor must be (n_operators, dim, dim)");
+            hip::check(
+                fp_hip_summed_to_tensor(device_ops(), hip::dtype_of<T>(), A_k_out.data_handle(), dim(), 1, nullptr));
+            hip::check(fp_hip_stream_synchronize(nullptr));
+            return;
+        }
         for (std::size_t k = 0; k < n_operators(); ++k)
             for (std::size_t i = 0; i < n_pauli_strings(); ++i)
             {

@@ -169,6 +169,44 @@ int main()
                 CHECK(std::abs(acc - std::complex<double>(out[i * S + t])) < 1e-6);
             }
     }
+    // to_tensor into device memory runs on the device and equals the host form element by element
+    {
+        PauliString ps("XYZIY");
+        std::size_t const d = ps.dim();
+        std::vector<cd> host(d * d, cd(0.25, -0.5)), dev(d * d);
+        hip::Buffer buf(d * d * sizeof(cd));
+        hip::h2d(buf.get(), host.data(), d * d * sizeof(cd));
+        ps.to_tensor<double>(mdspan(host.data(), d, d));
+        ps.to_tensor<double>(mdspan(static_cast<cd *>(buf.get()), d, d));
+        hip::d2h(dev.data(), buf.get(), d * d * sizeof(cd));
+        for (std::size_t i = 0; i < d * d; ++i)
+            CHECK(dev[i] == host[i]);
+
+        std::vector<PauliString> const strs = {PauliString("XYZIY"), PauliString("ZZIXX"), PauliString("YZIXX"),
+                                               PauliString("IIIII")};
+        std::vector<cd> const cf = {cd(0.5, -1.0), cd(2.0, 0.0), cd(0.0, -0.75), cd(-1.25, 0.5)};
+        PauliOp<double> op(cf, strs);
+        std::vector<cd> h2 = rand_vec(d * d, 7), d2(d * d);
+        hip::Buffer b2(d * d * sizeof(cd));
+        hip::h2d(b2.get(), h2.data(), d * d * sizeof(cd));
+        op.to_tensor(mdspan(h2.data(), d, d));
+        op.to_tensor(mdspan(static_cast<cd *>(b2.get()), d, d));
+        hip::d2h(d2.data(), b2.get(), d * d * sizeof(cd));
+        for (std::size_t i = 0; i < d * d; ++i)
+            CHECK(d2[i] == h2[i]);
+
+        std::size_t const K = 3;
+        auto const hk = rand_vec(strs.size() * K, 9);
+        SummedPauliOp<double> sop(strs, hk); // coefficients (n_strings x K), row-major
+        std::vector<cd> h3(K * d * d, cd(0.0, 0.0)), d3(K * d * d);
+        hip::Buffer b3(K * d * d * sizeof(cd));
+        hip::h2d(b3.get(), h3.data(), K * d * d * sizeof(cd));
+        sop.to_tensor(mdspan(h3.data(), K, d, d));
+        sop.to_tensor(mdspan(static_cast<cd *>(b3.get()), K, d, d));
+        hip::d2h(d3.data(), b3.get(), K * d * d * sizeof(cd));
+        for (std::size_t i = 0; i < K * d * d; ++i)
+            CHECK(d3[i] == h3[i]);
+    }
     std::printf("host API checks: %s (%d failures)\n", failures ? "FAILED" : "passed", failures);
     return failures ? 1 : 0;
 }
