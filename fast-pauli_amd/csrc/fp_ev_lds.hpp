@@ -193,13 +193,37 @@ __device__ __forceinline__ void chunk_rows_n(typename fpk::VecOf<T, 1>::type con
 #pragma unroll
         for (int s = 0; s < CNT; ++s)
             flip[s] = uint32_t(__popc(uq0 & zhi[s]) & 1) << 31;
-#pragma unroll
-        for (int k = 0; k < U; ++k)
+        if constexpr (sizeof(A) == 4)
         {
-            A const g = g_of(a[k], b[k]);
+            // FP32: the U terms of a string first, with its per-chunk signs, then this
+            // step's sign once: one scalar sign per string instead of one per term
+            // (the FP32 form is bound by scalar instructions, FP64 by the vector FMAs)
+            A tsum[CNT];
 #pragma unroll
             for (int s = 0; s < CNT; ++s)
-                acc[s] = fma(xor_sign(sgk[s][k], flip[s]), g, acc[s]);
+                tsum[s] = A(0);
+#pragma unroll
+            for (int k = 0; k < U; ++k)
+            {
+                A const g = g_of(a[k], b[k]);
+#pragma unroll
+                for (int s = 0; s < CNT; ++s)
+                    tsum[s] = fma(sgk[s][k], g, tsum[s]);
+            }
+#pragma unroll
+            for (int s = 0; s < CNT; ++s)
+                acc[s] = fma(xor_sign(A(1), flip[s]), tsum[s], acc[s]);
+        }
+        else
+        {
+#pragma unroll
+            for (int k = 0; k < U; ++k)
+            {
+                A const g = g_of(a[k], b[k]);
+#pragma unroll
+                for (int s = 0; s < CNT; ++s)
+                    acc[s] = fma(xor_sign(sgk[s][k], flip[s]), g, acc[s]);
+            }
         }
     }
     for (uint32_t q = n_full; q < n_iter; ++q) // fewer than U row blocks left
