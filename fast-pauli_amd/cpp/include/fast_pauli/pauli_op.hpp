@@ -15,6 +15,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "device.hpp"
@@ -167,25 +168,50 @@ template <std::floating_point T, typename H = std::complex<T>> struct PauliOp
             throw std::invalid_argument("Mismatched dimensions for provided PauliOp");
         std::vector<H> c;
         std::vector<PauliString> s;
-        std::map<std::string, std::size_t> where;
-        for (std::size_t i = 0; i < lhs.n_pauli_strings(); ++i)
-            for (std::size_t j = 0; j < rhs.n_pauli_strings(); ++j)
+        auto add = [&](auto &where, auto const &key, std::complex<double> phase, PauliString &&p, std::size_t i,
+                       std::size_t j) {
+            H const cij = H(phase) * lhs.coeffs[i] * rhs.coeffs[j];
+            auto it = where.find(key);
+            if (it == where.end())
             {
-                auto [phase, p] = lhs.pauli_strings[i] * rhs.pauli_strings[j];
-                H const cij = H(phase) * lhs.coeffs[i] * rhs.coeffs[j];
-                auto const key = p.str();
-                auto it = where.find(key);
-                if (it == where.end())
-                {
-                    where.emplace(key, s.size());
-                    s.push_back(std::move(p));
-                    c.push_back(cij);
-                }
-                else
-                {
-                    c[it->second] += cij;
-                }
+                where.emplace(key, s.size());
+                s.push_back(std::move(p));
+                c.push_back(cij);
             }
+            else
+            {
+                c[it->second] += cij;
+            }
+        };
+        std::size_t const n = lhs.n_qubits();
+        if (n <= 32)
+        {
+            // products keyed by their (x, z) bit masks (letter order is fixed by n)
+            std::unordered_map<std::uint64_t, std::size_t> where;
+            for (std::size_t i = 0; i < lhs.n_pauli_strings(); ++i)
+                for (std::size_t j = 0; j < rhs.n_pauli_strings(); ++j)
+                {
+                    auto [phase, p] = lhs.pauli_strings[i] * rhs.pauli_strings[j];
+                    std::uint64_t x = 0, z = 0;
+                    for (std::size_t q = 0; q < n; ++q)
+                    {
+                        x |= std::uint64_t(p.paulis[q].code == 1 || p.paulis[q].code == 2) << q;
+                        z |= std::uint64_t(p.paulis[q].code == 2 || p.paulis[q].code == 3) << q;
+                    }
+                    add(where, (x << 32) | z, phase, std::move(p), i, j);
+                }
+        }
+        else
+        {
+            std::map<std::string, std::size_t> where;
+            for (std::size_t i = 0; i < lhs.n_pauli_strings(); ++i)
+                for (std::size_t j = 0; j < rhs.n_pauli_strings(); ++j)
+                {
+                    auto [phase, p] = lhs.pauli_strings[i] * rhs.pauli_strings[j];
+                    std::string const key = p.str();
+                    add(where, key, phase, std::move(p), i, j);
+                }
+        }
         return PauliOp(std::move(c), std::move(s));
     }
 
