@@ -295,6 +295,6 @@ def test_planted_mistakes_are_caught():
     k, m = O.sparse_repr("XYZI")
     wrong_m = np.conj(m)
     assert not np.allclose(O.dense_string("XYZI")[np.arange(16), k.astype(int)], wrong_m)
-    # the survey's literal i^{nY} (-1)^{popcount(j&z)} on the partner column form
+    # the literal form i^{nY} (-1)^{popcount(j&z)} (sign of the phase flipped for odd nY)
     kc, mc = closed_form("YII")
     assert not np.allclose(np.conj(mc), mc)
