@@ -16,12 +16,23 @@ value = strings x states per second over the whole step = (1 + 256) * 16 / t_ste
 
 Roofline: algorithmic HBM bytes of each kernel launch divided by its average
 duration, measured with HIP events on the stream the kernels run on:
-  expectation_value: N * 2^n * S * 16 B   (each string reads the state once)
-  apply_batch:       2 * 2^n * S * 16 B   (read psi, write P psi)
+  expectation_value (coset-tiled, the default): groups x 2^n x S x 16 B
+                     (each group of strings reads the state once; the plan
+                     reports the group count)
+  expectation_value (streaming form, reported beside it): N x 2^n x S x 16 B
+  apply_batch:       2 x 2^n x S x 16 B   (read psi, write P psi)
 against the MI355X HBM3E peak of 8.0 TB/s.
 
+Parity is checked in the same run: the default expectation kernel against the
+streaming kernel (<= 1e-12), the complex64 form against FP64 arithmetic on the
+same inputs (<= 1e-5), and, on the CPU baseline's sample of strings, both
+precisions against the oracle (<= 1e-12 / <= 1e-5) and apply_batch against the
+oracle (IEEE-equal).  The script exits with status 1 when a check fails.
+
 The CPU baseline is the repository's OpenMP restatement of the reference's
-loops (oracle/), timed on this host on a bounded sample of the same workload.
+loops (oracle/), timed on this host on a bounded sample of the same workload
+with OMP_PROC_BIND=close and every available core unless OMP_NUM_THREADS says
+otherwise.
 """
 
 from __future__ import annotations
@@ -40,6 +51,7 @@ for _p in (ROOT, os.path.join(ROOT, "fast-pauli_amd")):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0
+AFFINITY_CPUS = None  # CPUs this process may run on, read at start-up
 
 
 def log(msg: str) -> None:
@@ -61,6 +73,8 @@ def parse():
                     help="skip the other workloads (n=10 PauliOp.apply, n=14 SummedPauliOp, uneven batch, serving)")
     ap.add_argument("--qps", type=float, default=50.0, help="serving: Poisson arrival rate")
     ap.add_argument("--serve-requests", type=int, default=300, help="serving: number of requests")
+    ap.add_argument("--qps-sweep", default="50,150,300,450,600",
+                    help="serving: comma-separated arrival rates for the saturation sweep ('' to skip)")
     ap.add_argument("--pmc-pass", action="store_true",
                     help="only run the profiled kernels once each (a plain copy of the state, apply_batch, "
                          "expectation_value) for hardware-counter runs")
@@ -90,6 +104,12 @@ def pmc_gemm() -> int:
 
 def main() -> int:
     args = parse()
+    # the CPU baseline's OpenMP team: every available core unless the caller says otherwise,
+    # threads bound close (read when the oracle library loads)
+    global AFFINITY_CPUS
+    AFFINITY_CPUS = len(os.sched_getaffinity(0))  # before OpenMP binds this thread to one place
+    os.environ.setdefault("OMP_NUM_THREADS", str(AFFINITY_CPUS))
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     if args.pmc_gemm:
         return pmc_gemm()
     if args.gpus != 1:
@@ -193,6 +213,10 @@ def main() -> int:
     bytes_ev_stream = N * dim * S * bc
     log(f"complex128 streaming EV kernel: {ms_ev_stream:.3f} ms")
 
+    failures: list[str] = []
+    rel_stream = float(np.linalg.norm(ev_stream - ev128) / np.linalg.norm(ev128))
+    if not rel_stream <= 1e-12:
+        failures.append(f"default expectation kernel differs from the streaming kernel by {rel_stream:.3e} > 1e-12")
     result = {
         "metric": "strings\u00d7states/s & % HBM peak at n=24; p50/p99 latency at fixed QPS",
         "value": value,
@@ -238,7 +262,7 @@ def main() -> int:
             "algorithmic_bytes_per_launch": bytes_ev_stream,
             "avg_launch_ms": ms_ev_stream,
             "speedup_of_tiled": ms_ev_stream / ms_ev,
-            "rel_diff_vs_default_kernel": float(np.linalg.norm(ev_stream - ev128) / np.linalg.norm(ev128)),
+            "rel_diff_vs_default_kernel": rel_stream,
         },
         "roofline_apply_batch": {
             "kernel": "k_string_apply (PauliString::apply_batch)",
@@ -288,11 +312,19 @@ def main() -> int:
         }
         log(f"complex64: step {ms_step64:.3f} ms (apply {ms_apply64:.3f}, EV {ms_ev64:.3f}); "
             f"EV rel err {err_norm:.2e}")
-        del psi64, out64, ev64
+        if not err_norm <= 1e-5:
+            failures.append(f"complex64 expectation differs from FP64 on the same inputs by {err_norm:.3e} > 1e-5")
+        del out64, ev64
+    else:
+        psi64 = None
 
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(args, psi, codes_apply, codes_ev, coeffs_ev)
-    del psi, out
+        cpu, apply_ref, ev_ref, n_sample = cpu_baseline(args, psi, codes_apply, codes_ev, coeffs_ev)
+        result["cpu_baseline"] = cpu
+        result["parity_vs_oracle"] = oracle_parity(args, psi, out, psi64, ops_apply, codes_ev, coeffs_ev, apply_ref,
+                                                   ev_ref, n_sample, failures)
+        del apply_ref
+    del psi, out, psi64
 
     if not args.no_extra:
         result["workloads"] = {
@@ -307,7 +339,12 @@ def main() -> int:
         result["p99_ms"] = serving["p99_ms"]
         result["serving_qps"] = serving["offered_qps"]
 
+    result["parity_checks_failed"] = failures
     print(json.dumps(result), flush=True)
+    if failures:
+        for f in failures:
+            log(f"PARITY FAILURE: {f}")
+        return 1
     return 0
 
 
@@ -515,29 +552,59 @@ def bench_uneven_batch(args) -> dict:
 
 
 def bench_serving(args) -> dict:
-    """Poisson request stream: 70% expectation_value / 30% apply, n 10..22, 8..1024 strings, 1..32 states."""
+    """Poisson request stream: 70% expectation_value / 30% apply, n 10..22, 8..1024 strings, 1..32 states.
+
+    The headline point runs at --qps; the sweep (--qps-sweep) runs the same request
+    list at each rate on a fresh server and reports latency, mean batch size and
+    whether the loop kept up (achieved >= 0.9 x offered)."""
     from fast_pauli.serving import Server, request_mix, run_poisson
 
-    log(f"workload: serving, {args.serve_requests} requests at {args.qps} QPS")
+    cfg = dict(hbm_budget_bytes=200 << 30, n_slots=4, max_batch=32)
     reqs = request_mix(args.serve_requests, seed=args.seed)
-    with Server(hbm_budget_bytes=200 << 30, n_slots=4, max_batch=32) as srv:
-        out = run_poisson(srv, reqs, qps=args.qps, seed=args.seed)
+
+    def point(qps):
+        with Server(**cfg) as srv:
+            r = run_poisson(srv, reqs, qps=qps, seed=args.seed)
+        st = r["stats"]
+        r["mean_batch_size"] = st["batched_requests"] / max(st["batches"], 1)
+        r["kept_up"] = r["achieved_qps"] >= 0.9 * qps
+        return r
+
+    log(f"workload: serving, {args.serve_requests} requests at {args.qps} QPS")
+    out = point(args.qps)
+    sweep = []
+    for q in [float(v) for v in args.qps_sweep.split(",") if v.strip()]:
+        log(f"workload: serving sweep, {args.serve_requests} requests at {q} QPS")
+        r = point(q)
+        sweep.append({k: r[k] for k in ("offered_qps", "achieved_qps", "p50_ms", "p99_ms", "queue_p50_ms",
+                                        "mean_batch_size", "kept_up", "failed", "strings_x_states_per_s")})
+        sweep[-1]["batches"] = r["stats"]["batches"]
+        sweep[-1]["max_concurrent_batches"] = r["stats"]["max_concurrent_batches"]
+    out["sweep"] = sweep
+    kept = [p["offered_qps"] for p in sweep if p["kept_up"]]
+    out["saturation_qps"] = {"highest_rate_kept_up": max(kept) if kept else None,
+                             "lowest_rate_fell_behind": min((p["offered_qps"] for p in sweep if not p["kept_up"]),
+                                                            default=None)}
     out["config"] = ("Poisson arrivals, 70% PauliOp.expectation_value / 30% PauliOp.apply, n ~ U{10..22}, "
                      "strings ~ U{8..1024}, states ~ U{1..32}, complex64 device-resident, 200 GB admission budget, "
-                     "4 concurrent batch slots")
+                     "4 concurrent batch slots, up to 32 requests per batch")
     return out
 
 
-def cpu_baseline(args, psi_dev, codes_apply, codes_ev, coeffs_ev) -> dict:
-    """The reference's OpenMP loops (oracle/, 'par' branches) on this host, bounded sample."""
+def cpu_baseline(args, psi_dev, codes_apply, codes_ev, coeffs_ev):
+    """The reference's OpenMP loops (oracle/, 'par' branches) on this host, bounded sample.
+
+    Returns (record, apply output, expectation values of the sample, sample size): the
+    outputs are the oracle's, kept for the parity checks."""
     from oracle import oracle as O
     from fast_pauli import synthetic as syn
 
     n, S, N = args.n, args.states, args.strings
     dim = 1 << n
     threads = O.max_threads()
-    n_sample = int(min(N, max(16, threads)))
-    log(f"cpu baseline: {threads} OpenMP threads, apply_batch x1 + expectation_value x{n_sample} at n={n}")
+    n_sample = int(min(N, max(32, threads)))
+    log(f"cpu baseline: {threads} OpenMP threads (OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}), "
+        f"apply_batch x1 + expectation_value x{n_sample} at n={n}")
     psi = psi_dev.to_array(np.complex128, (dim, S))
     s_apply = syn.codes_to_strings(codes_apply)[0]
     strings = syn.codes_to_strings(codes_ev[:n_sample])
@@ -545,7 +612,6 @@ def cpu_baseline(args, psi_dev, codes_apply, codes_ev, coeffs_ev) -> dict:
     t = time.perf_counter()
     O.string_apply_batch(s_apply, 1, psi, out, par=True)
     t_apply = time.perf_counter() - t
-    del out
     evo = np.zeros(S, np.complex128)
     t = time.perf_counter()
     O.op_expectation(strings, coeffs_ev[:n_sample], psi, evo, par=True)
@@ -560,19 +626,76 @@ def cpu_baseline(args, psi_dev, codes_apply, codes_ev, coeffs_ev) -> dict:
                     break
     except OSError:
         pass
-    return {
+    record = {
         "value": (1 + N) * S / t_step,
         "unit": "strings*states/s",
         "cores": threads,
         "kind": "port",
         "sample": f"n={n}, {S} states (same device-generated states), apply_batch of 1 string "
                   f"({t_apply:.2f} s) + PauliOp.expectation_value of {n_sample} of the {N} strings "
-                  f"({t_ev:.2f} s), par (OpenMP) branches of the reference loops; EV time scaled "
-                  f"by {N}/{n_sample} to the full step",
+                  f"({t_ev:.2f} s), par (OpenMP) branches of the reference loops; strings run "
+                  f"independently per thread, so the EV time is scaled by {N}/{n_sample} to the full step",
         "cpu_model": cpu_model,
+        "omp_num_threads": threads,
+        "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+        "nproc": os.cpu_count(),
+        "affinity_cpus": AFFINITY_CPUS,
         "apply_batch_s": t_apply,
         "expectation_value_s_per_string": t_ev / n_sample,
     }
+    return record, out, evo, n_sample
+
+
+def oracle_parity(args, psi_dev, out_dev, psi64_dev, ops_apply, codes_ev, coeffs_ev, apply_ref, ev_ref, n_sample,
+                  failures) -> dict:
+    """The GPU results on the CPU baseline's sample against the oracle's outputs:
+    apply_batch IEEE-equal; PauliOp.expectation_value of the sample's strings
+    <= 1e-12 (complex128) and <= 1e-5 (complex64, against the oracle in FP64 on the
+    same complex64 inputs, and against the complex128 oracle result)."""
+    from fast_pauli import _abi
+    from fast_pauli import synthetic as syn
+    from oracle import oracle as O
+
+    n, S = args.n, args.states
+    dim = 1 << n
+    call = _abi.call
+    res = {"sample_strings": n_sample}
+    call("fp_hip_string_apply_batch", ops_apply.handle, 0, _abi.FP_C128, psi_dev.ptr, out_dev.ptr, dim, S,
+         1.0, 0.0, 0, None)
+    got = out_dev.to_array(np.complex128, (dim, S))
+    res["apply_batch_ieee_equal"] = bool(np.array_equal(got, apply_ref))
+    del got
+    if not res["apply_batch_ieee_equal"]:
+        failures.append("apply_batch differs from the oracle")
+    ops = _abi.Ops(codes_ev[:n_sample], coeffs_ev[:n_sample])
+    evb = _abi.DeviceBuffer(S * 16)
+    plan = _abi.op_expectation_plan(ops, _abi.FP_C128, dim, S)
+    call("fp_hip_op_expectation", ops.handle, _abi.FP_C128, psi_dev.ptr, evb.ptr, dim, S, 0, None)
+    g128 = evb.to_array(np.complex128, (S,))
+    scale = np.abs(coeffs_ev[:n_sample]).sum()
+    e128 = float(np.linalg.norm(g128 - ev_ref) / np.linalg.norm(ev_ref))
+    res["ev_c128"] = {"kernel": plan["kernel"], "groups": plan["n_groups"], "rel_err_vs_oracle": e128,
+                      "max_abs_err_over_sum_abs_coeffs": float(np.abs(g128 - ev_ref).max() / scale)}
+    if not e128 <= 1e-12:
+        failures.append(f"complex128 expectation differs from the oracle by {e128:.3e} > 1e-12")
+    log(f"parity: apply_batch IEEE-equal {res['apply_batch_ieee_equal']}, complex128 EV x{n_sample} "
+        f"rel err vs oracle {e128:.2e}")
+    if psi64_dev is not None:
+        call("fp_hip_op_expectation", ops.handle, _abi.FP_C64, psi64_dev.ptr, evb.ptr, dim, S, 0, None)
+        g64 = evb.to_array(np.complex64, (S,)).astype(np.complex128)
+        psi64w = psi64_dev.to_array(np.complex64, (dim, S)).astype(np.complex128)
+        ref64 = np.zeros(S, np.complex128)
+        O.op_expectation(syn.codes_to_strings(codes_ev[:n_sample]),
+                         coeffs_ev[:n_sample].astype(np.complex64).astype(np.complex128), psi64w, ref64, par=True)
+        del psi64w
+        e_same = float(np.linalg.norm(g64 - ref64) / np.linalg.norm(ref64))
+        e_fp64 = float(np.linalg.norm(g64 - ev_ref) / np.linalg.norm(ev_ref))
+        res["ev_c64"] = {"rel_err_vs_oracle_same_inputs": e_same, "rel_err_vs_oracle_complex128_inputs": e_fp64,
+                         "max_abs_err_over_sum_abs_coeffs": float(np.abs(g64 - ev_ref).max() / scale)}
+        if not e_same <= 1e-5:
+            failures.append(f"complex64 expectation differs from the oracle by {e_same:.3e} > 1e-5")
+        log(f"parity: complex64 EV rel err vs oracle {e_same:.2e} (same inputs), {e_fp64:.2e} (complex128 inputs)")
+    return res
 
 
 if __name__ == "__main__":

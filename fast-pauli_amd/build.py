@@ -46,10 +46,21 @@ def _run(cmd: list[str]) -> None:
 
 
 def build_hip() -> str:
+    """Compiles every translation unit in parallel (each carries its own gfx950
+    code object; no kernel is launched from another unit) and links the library."""
+    from concurrent.futures import ThreadPoolExecutor
+
     srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+    objs = [os.path.join(CSRC, os.path.basename(src)[:-4] + ".o") for src in srcs]
+    jobs = max(1, min(len(srcs), int(os.environ.get("MAX_JOBS", "0") or 0) or (os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        list(pool.map(lambda so: _run([HIPCC, *HIP_FLAGS, "-c", f"-I{INCLUDE}", f"-I{CSRC}", "-o", so[1], so[0]]),
+                      zip(srcs, objs)))
     tmp = LIB_HIP + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", f"-I{INCLUDE}", f"-I{CSRC}", "-o", tmp, *srcs])
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *objs])
     os.replace(tmp, LIB_HIP)
+    for o in objs:
+        os.remove(o)
     return LIB_HIP
 
 

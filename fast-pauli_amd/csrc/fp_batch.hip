@@ -29,6 +29,7 @@ struct BatchEntry
     size_t dim;
     size_t n_states;
     int accumulate;
+    uint64_t generation; // ops->generation the device tables were built from
 };
 
 struct fp_hip_batch
@@ -162,8 +163,14 @@ fp_status build(fp_hip_batch *b, hipStream_t stream)
     fp_hip_batch_schedule const &sc = b->sched;
     for (size_t i = 0; i < b->entries.size(); ++i)
     {
-        BatchEntry const &e = b->entries[i];
+        BatchEntry &e = b->entries[i];
         fp_hip_ops const *o = e.ops;
+        // the operator set may have been re-targeted since the entry was added
+        if (e.dim != (size_t(1) << o->n_qubits))
+            return fail(FP_ERR_INVALID_ARGUMENT,
+                        "batch entry %zu: its operator set now acts on %zu qubits, not on the entry's %zu rows", i,
+                        o->n_qubits, e.dim);
+        e.generation = o->generation;
         fprt::VecGeom const g = fprt::vec_geom(e.dtype, e.dim, e.n_states, e.states, e.out);
         fpb::BatchOp op{};
         op.in = e.states;
@@ -335,7 +342,7 @@ fp_status fp_hip_batch_add(fp_hip_batch *b, int kind, fp_hip_ops const *ops, int
     uintptr_t const align = dtype == FP_C128 ? 16 : 8;
     if (((reinterpret_cast<uintptr_t>(states) | reinterpret_cast<uintptr_t>(out)) & (align - 1)) != 0)
         return fail(FP_ERR_INVALID_ARGUMENT, "device buffers must be aligned to the complex element size");
-    BatchEntry const e{ops, kind, dtype, states, out, dim, n_states, accumulate};
+    BatchEntry const e{ops, kind, dtype, states, out, dim, n_states, accumulate, ops->generation};
     // an output may not overlap any other entry's input or output (they run concurrently)
     for (BatchEntry const &o : b->entries)
     {
@@ -394,6 +401,14 @@ fp_status fp_hip_batch_launch(fp_hip_batch *b, void *stream)
     if (!b)
         return fail(FP_ERR_INVALID_ARGUMENT, "batch is NULL");
     hipStream_t const st = static_cast<hipStream_t>(stream);
+    // tables and graphs hold the operator sets' device pointers and string counts:
+    // rebuild when a set has been re-targeted (or given new coefficients) since
+    for (BatchEntry const &e : b->entries)
+        if (b->built && e.ops->generation != e.generation)
+        {
+            wait_idle(b);
+            drop_graph(b);
+        }
     if (!b->built)
     {
         fp_status const r = build(b, st);

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <string>
@@ -41,8 +42,9 @@ fp_status fail(fp_status code, char const *fmt, ...)
 
 // ---------------------------------------------------------------------------
 // Per-stream scratch: reduction partials etc.  Grown on demand (after the
-// stream has drained), never shrunk.  Calls on one stream are ordered, so one
-// buffer per stream suffices.
+// stream has drained), never shrunk.  Calls on one stream are ordered, and the
+// entry points that use scratch hold the stream's StreamGuard until their
+// kernels are enqueued, so one buffer per stream suffices.
 // ---------------------------------------------------------------------------
 struct Scratch
 {
@@ -51,6 +53,26 @@ struct Scratch
 };
 std::mutex g_scratch_mu;
 std::map<std::pair<int, hipStream_t>, Scratch> g_scratch;
+std::map<std::pair<int, hipStream_t>, std::unique_ptr<std::recursive_mutex>> g_stream_mu;
+
+StreamGuard::StreamGuard(hipStream_t stream)
+{
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        auto &m = g_stream_mu[{dev, stream}];
+        if (!m)
+            m = std::make_unique<std::recursive_mutex>();
+        mu_ = m.get();
+    }
+    mu_->lock();
+}
+
+StreamGuard::~StreamGuard()
+{
+    mu_->unlock();
+}
 
 fp_status scratch_get(hipStream_t stream, size_t bytes, void **out)
 {
@@ -62,6 +84,8 @@ fp_status scratch_get(hipStream_t stream, size_t bytes, void **out)
     {
         if (s.ptr)
         {
+            // earlier calls on this stream have enqueued all their kernels (the
+            // caller holds the stream's guard): once they finish, the buffer is free
             FP_HIP(hipStreamSynchronize(stream));
             FP_HIP(hipFree(s.ptr));
             s.ptr = nullptr;
@@ -309,6 +333,7 @@ fp_status ops_assign(fp_hip_ops *o, size_t n_qubits, size_t n_strings, uint8_t c
         if (codes[i] > 3)
             return fail(FP_ERR_INVALID_ARGUMENT, "Pauli code must be 0, 1, 2, or 3 (got %u)", unsigned(codes[i]));
     fprt::invalidate_tiles(o); // string groups belong to the previous strings
+    ++o->generation;
     ArenaLayout const L = arena_layout(n_strings);
     if (o->upload_done)
         FP_HIP(hipEventSynchronize(o->upload_done)); // the previous upload may still read h_arena
@@ -381,6 +406,11 @@ fp_status ops_assign(fp_hip_ops *o, size_t n_qubits, size_t n_strings, uint8_t c
     return FP_OK;
 }
 
+} // namespace
+
+namespace fprt
+{
+
 fp_status check_ops(fp_hip_ops const *ops)
 {
     if (!ops)
@@ -409,6 +439,27 @@ fp_status check_shape(fp_hip_ops const *ops, int dtype, void const *a, void cons
     if (((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & (align - 1)) != 0)
         return fail(FP_ERR_INVALID_ARGUMENT, "device buffers must be aligned to the complex element size (%zu bytes)",
                     size_t(align));
+    return FP_OK;
+}
+
+} // namespace fprt
+
+namespace
+{
+
+bool ranges_overlap(void const *a, size_t na, void const *b, size_t nb)
+{
+    auto const pa = reinterpret_cast<uintptr_t>(a), pb = reinterpret_cast<uintptr_t>(b);
+    return pa < pb + nb && pb < pa + na;
+}
+
+// An apply writes `out` while other threads still read `states`: the byte ranges
+// of the two (dim x n_states) batches must be disjoint.
+fp_status check_no_alias(int dtype, void const *states, void const *out, size_t dim, size_t n_states)
+{
+    size_t const bytes = dim * n_states * (dtype == FP_C128 ? 16 : 8);
+    if (ranges_overlap(states, bytes, out, bytes))
+        return fail(FP_ERR_INVALID_ARGUMENT, "apply cannot run in place (out overlaps states)");
     return FP_OK;
 }
 
@@ -549,19 +600,54 @@ void launch_strings_expect(VecGeom const &vg, EvGeom const &e, void const *state
                        n_states, partial);
 }
 
+// Shrinks an expectation grid to at most `max_parts` partials per state: fewer
+// string blocks first, then fewer row blocks.  Every (pair, string) stays covered.
+void fit_ev_geom(EvGeom &e, size_t count, uint64_t max_parts)
+{
+    if (uint64_t(e.gx) * e.gz <= max_parts)
+        return;
+    uint64_t gz = std::max<uint64_t>(1, max_parts / e.gx);
+    e.spb = uint32_t((count + gz - 1) / gz);
+    e.gz = uint32_t((count + e.spb - 1) / e.spb);
+    if (uint64_t(e.gx) * e.gz > max_parts)
+    {
+        uint64_t const gx = std::max<uint64_t>(1, max_parts / e.gz);
+        e.ppb = (e.n_pairs + gx - 1) / gx;
+        e.gx = uint32_t((e.n_pairs + e.ppb - 1) / e.ppb);
+    }
+}
+
+} // namespace
+
+namespace fprt
+{
+
 // Expectation values of strings [s_begin, s_begin + count) combined with
 // per-string complex scales, reduced to out[t] (+)= post * sum.
 fp_status run_op_expect(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim, size_t n_states,
                         size_t s_begin, size_t count, double const *scale_table, double post_re, double post_im,
-                        int accumulate, hipStream_t st)
+                        int accumulate, hipStream_t st, Workspace const *ws)
 {
     VecGeom const vg = vec_geom(dtype, dim, n_states, states, states);
-    EvGeom const e = ev_geom(vg, dim, count, false, n_states);
-    uint64_t const n_parts = uint64_t(e.gx) * e.gz;
+    EvGeom e = ev_geom(vg, dim, count, false, n_states);
     void *scratch = nullptr;
-    fp_status s = scratch_get(st, n_parts * n_states * sizeof(fpk::Dbl2), &scratch);
-    if (s != FP_OK)
-        return s;
+    if (ws)
+    {
+        uint64_t const per_part = uint64_t(n_states) * sizeof(fpk::Dbl2);
+        if (per_part > ws->bytes)
+            return fail(FP_ERR_INVALID_ARGUMENT,
+                        "%zu states need %llu bytes of expectation partials, more than the %zu-byte workspace",
+                        n_states, static_cast<unsigned long long>(per_part), ws->bytes);
+        fit_ev_geom(e, count, ws->bytes / per_part);
+        scratch = ws->base;
+    }
+    uint64_t const n_parts = uint64_t(e.gx) * e.gz;
+    if (!ws)
+    {
+        fp_status const s = scratch_get(st, n_parts * n_states * sizeof(fpk::Dbl2), &scratch);
+        if (s != FP_OK)
+            return s;
+    }
     auto *partial = static_cast<fpk::Dbl2 *>(scratch);
     fpk::StringMeta const *meta = ops->d_meta + s_begin;
     double const *scale = scale_table + 2 * s_begin;
@@ -582,6 +668,11 @@ fp_status run_op_expect(fp_hip_ops const *ops, int dtype, void const *states, vo
     FP_LAUNCHED();
     return FP_OK;
 }
+
+} // namespace fprt
+
+namespace
+{
 
 // The state column fits in a workgroup's LDS: the x-grouped LDS kernel applies.
 bool lds_eligible(fp_hip_ops const *ops, int dtype, size_t dim)
@@ -912,6 +1003,7 @@ fp_status fp_hip_fill_states(int dtype, void *states, size_t dim, size_t n_state
     if (dtype != FP_C128 && dtype != FP_C64)
         return fail(FP_ERR_INVALID_ARGUMENT, "unknown dtype %d", dtype);
     hipStream_t const st = static_cast<hipStream_t>(stream);
+    StreamGuard guard(st);
     uint64_t const n_elem = uint64_t(dim) * n_states;
     uint64_t const n_real = 2 * n_elem;
     uint32_t const blocks = uint32_t((n_real + fpk::kBlock - 1) / fpk::kBlock);
@@ -1074,6 +1166,7 @@ fp_status fp_hip_ops_set_coeffs(fp_hip_ops *ops, double const *coeffs)
         return fail(FP_ERR_INVALID_ARGUMENT, "coeffs is NULL");
     (void)hipDeviceSynchronize();
     fprt::invalidate_tiles(ops); // the tiled expectation tables hold the scaled coefficients
+    ++ops->generation;
     std::memcpy(ops->coeffs.data(), coeffs, 2 * ops->n_strings * sizeof(double));
     ArenaLayout const L = arena_layout(ops->n_strings);
     fill_coeff_tables(ops, L);
@@ -1089,6 +1182,7 @@ fp_status fp_hip_ops_set_matrix(fp_hip_ops *ops, size_t n_operators, double cons
     if (n_operators == 0 || !coeffs_nk)
         return fail(FP_ERR_INVALID_ARGUMENT, "empty coefficient matrix");
     (void)hipDeviceSynchronize();
+    ++ops->generation;
     size_t const N = ops->n_strings, K = n_operators;
     ops->n_operators = K;
     ops->matrix.assign(coeffs_nk, coeffs_nk + 2 * N * K);
@@ -1174,8 +1268,8 @@ fp_status fp_hip_string_apply_batch(fp_hip_ops const *ops, size_t string_index, 
     if (string_index >= ops->n_strings)
         return fail(FP_ERR_INVALID_ARGUMENT, "string index %zu out of range (%zu strings)", string_index,
                     ops->n_strings);
-    if (states == out)
-        return fail(FP_ERR_INVALID_ARGUMENT, "apply cannot run in place (out aliases states)");
+    if ((st = check_no_alias(dtype, states, out, dim, n_states)) != FP_OK)
+        return st;
     hipStream_t const s = static_cast<hipStream_t>(stream);
     VecGeom const g = vec_geom(dtype, dim, n_states, states, out);
     // fold (-i)^{nY} into the coefficient; exact
@@ -1208,6 +1302,7 @@ fp_status fp_hip_string_expectation(fp_hip_ops const *ops, size_t string_index, 
     if (string_index >= ops->n_strings)
         return fail(FP_ERR_INVALID_ARGUMENT, "string index %zu out of range (%zu strings)", string_index,
                     ops->n_strings);
+    StreamGuard guard(static_cast<hipStream_t>(stream));
     return run_op_expect(ops, dtype, states, out, dim, n_states, string_index, 1, ops->d_kappa_c, c_re, c_im,
                          accumulate, static_cast<hipStream_t>(stream));
 }
@@ -1218,8 +1313,8 @@ fp_status fp_hip_op_apply(fp_hip_ops const *ops, int dtype, void const *states, 
     fp_status st = check_shape(ops, dtype, states, out, dim, n_states);
     if (st != FP_OK)
         return st;
-    if (states == out)
-        return fail(FP_ERR_INVALID_ARGUMENT, "apply cannot run in place (out aliases states)");
+    if ((st = check_no_alias(dtype, states, out, dim, n_states)) != FP_OK)
+        return st;
     hipStream_t const s = static_cast<hipStream_t>(stream);
     bool tiled = false;
     if ((st = fprt::choose_op_apply_tiled(ops, dtype, n_states, &tiled)) != FP_OK)
@@ -1244,6 +1339,7 @@ fp_status fp_hip_op_expectation(fp_hip_ops const *ops, int dtype, void const *st
     fp_status st = check_shape(ops, dtype, states, out, dim, n_states);
     if (st != FP_OK)
         return st;
+    StreamGuard guard(static_cast<hipStream_t>(stream));
     bool tiled = false;
     if ((st = fprt::choose_op_expect_tiled(ops, dtype, n_states, &tiled)) != FP_OK)
         return st;
@@ -1260,6 +1356,7 @@ fp_status fp_hip_strings_expectation_matrix(fp_hip_ops const *ops, int dtype, vo
     fp_status st = check_shape(ops, dtype, states, ev, dim, n_states);
     if (st != FP_OK)
         return st;
+    StreamGuard guard(static_cast<hipStream_t>(stream));
     return run_strings_expect(ops, dtype, states, ev, dim, n_states, static_cast<hipStream_t>(stream), nullptr);
 }
 
@@ -1276,6 +1373,7 @@ fp_status fp_hip_summed_expectation(fp_hip_ops const *ops, int dtype, int contra
     if (contraction < FP_CONTRACT_FP64 || contraction > FP_CONTRACT_BF16X3)
         return fail(FP_ERR_INVALID_ARGUMENT, "unknown contraction mode %d", contraction);
     hipStream_t const s = static_cast<hipStream_t>(stream);
+    StreamGuard guard(s);
     size_t const N = ops->n_strings, K = ops->n_operators, S = n_states;
     bool const fp64 = contraction == FP_CONTRACT_FP64;
     size_t const esz = fp64 ? sizeof(double) : sizeof(float);
@@ -1356,8 +1454,8 @@ fp_status fp_hip_summed_apply(fp_hip_ops const *ops, int dtype, void const *stat
         return st;
     if (ops->n_operators == 0)
         return fail(FP_ERR_INVALID_ARGUMENT, "SummedPauliOp has no coefficient matrix");
-    if (states == out)
-        return fail(FP_ERR_INVALID_ARGUMENT, "apply cannot run in place (out aliases states)");
+    if ((st = check_no_alias(dtype, states, out, dim, n_states)) != FP_OK)
+        return st;
     hipStream_t const s = static_cast<hipStream_t>(stream);
     VecGeom const g = vec_geom(dtype, dim, n_states, states, out);
     uint32_t const N = uint32_t(ops->n_strings);
@@ -1381,9 +1479,13 @@ fp_status fp_hip_summed_apply_weighted(fp_hip_ops const *ops, int dtype, void co
         return fail(FP_ERR_INVALID_ARGUMENT, "SummedPauliOp has no coefficient matrix");
     if (!data)
         return fail(FP_ERR_INVALID_ARGUMENT, "data is NULL");
-    if (states == out)
-        return fail(FP_ERR_INVALID_ARGUMENT, "apply cannot run in place (out aliases states)");
+    if ((st = check_no_alias(dtype, states, out, dim, n_states)) != FP_OK)
+        return st;
+    if (ranges_overlap(data, ops->n_operators * n_states * (dtype == FP_C128 ? 8 : 4), out,
+                       dim * n_states * (dtype == FP_C128 ? 16 : 8)))
+        return fail(FP_ERR_INVALID_ARGUMENT, "out must not overlap data");
     hipStream_t const s = static_cast<hipStream_t>(stream);
+    StreamGuard guard(s);
     size_t const N = ops->n_strings, K = ops->n_operators, S = n_states;
     size_t const esz = dtype == FP_C128 ? sizeof(double) : sizeof(float);
     void *scratch = nullptr;
@@ -1459,6 +1561,7 @@ fp_status fp_hip_gemm(int contraction, size_t m, size_t n, size_t k, void const 
     if (!a || !b || !c || m == 0 || n == 0 || k == 0)
         return fail(FP_ERR_INVALID_ARGUMENT, "empty or NULL GEMM operand");
     hipStream_t const s = static_cast<hipStream_t>(stream);
+    StreamGuard guard(s);
     size_t const split_bytes = fpm::gemm_scratch_bytes(contraction, m, n, k);
     void *split = nullptr;
     fp_status st = FP_OK;

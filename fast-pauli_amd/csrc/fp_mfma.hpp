@@ -6,7 +6,7 @@
 // Three precisions:
 //   FP64   v_mfma_f64_16x16x4_f64          (plain / oracle mode)
 //   FP32   v_mfma_f32_16x16x4_f32
-//   BF16x3 v_mfma_f32_16x16x16_bf16: each FP32 operand is split into
+//   BF16x3 v_mfma_f32_16x16x32_bf16 (gfx950): each FP32 operand is split into
 //          hi = bf16(x), lo = bf16(x - hi); C = lo*hi + hi*lo + hi*hi with FP32
 //          accumulation (the dropped lo*lo term is ~2^-16 relative).
 //
@@ -18,9 +18,11 @@
 // fixed order by a second kernel, so results are deterministic.
 //
 // Operand layouts of the 16x16xK single-block instructions: lane l holds
-// A[l % 16][k0 + l/16] and B[k0 + l/16][l % 16] (the bf16 form: 4 consecutive
-// k per lane, k0 + 4*(l/16) + e); the f32 accumulator holds C[4*(l/16) + r][l % 16]
-// in register r, the f64 one C[l/16 + 4*r][l % 16].
+// A[l % 16][k0 + l/16] and B[k0 + l/16][l % 16] (the bf16 form: 8 consecutive
+// k per lane, k0 + 8*(l/16) + e); the f32 accumulator holds C[4*(l/16) + r][l % 16]
+// in register r, the f64 one C[l/16 + 4*r][l % 16].  A and B take their k runs
+// from the same LDS offsets, so a product term pairs A and B at one k whatever
+// order the instruction sums a lane's eight k in.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -41,7 +43,7 @@ inline char const *last_error()
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
-typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kBM = 64, kBN = 64;
 
@@ -86,7 +88,8 @@ template <> struct Traits<kBF16x3>
 
 // LDS slab: A as [BM][BK] (K contiguous), B as [BN][BK] (K contiguous, i.e.
 // transposed) so every lane reads its k-run contiguously.  BF16x3 stores hi and
-// lo planes of both.
+// lo planes of both, rows padded to 80 bytes: 16-byte aligned runs, and the 16
+// rows one read instruction touches start 20 dwords apart, on distinct banks.
 template <int MODE> struct Slab
 {
     typedef typename Traits<MODE>::T T;
@@ -97,8 +100,8 @@ template <int MODE> struct Slab
 template <> struct __align__(16) Slab<kBF16x3>
 {
     static constexpr int BK = Traits<kBF16x3>::BK;
-    uint16_t ah[kBM][BK + 4], al[kBM][BK + 4];
-    uint16_t bh[kBN][BK + 4], bl[kBN][BK + 4];
+    uint16_t ah[kBM][BK + 8], al[kBM][BK + 8];
+    uint16_t bh[kBN][BK + 8], bl[kBN][BK + 8];
 };
 
 template <int MODE>
@@ -183,29 +186,29 @@ __global__ __launch_bounds__(256) void k_gemm(typename Traits<MODE>::T const *__
         if constexpr (MODE == kBF16x3)
         {
 #pragma unroll
-            for (int kk = 0; kk < BK; kk += 16)
+            for (int kk = 0; kk < BK; kk += 32)
             {
-                uint32_t const kq = kk + 4 * (lane >> 4);
-                s16x4 ah[2], al[2], bh[2], bl[2];
+                uint32_t const kq = kk + 8 * (lane >> 4);
+                bf16x8 ah[2], al[2], bh[2], bl[2];
 #pragma unroll
                 for (int i = 0; i < 2; ++i)
                 {
                     uint32_t const ar = wm + 16 * i + (lane & 15u);
                     uint32_t const bc = wn + 16 * i + (lane & 15u);
-                    // rows are 72 bytes and kq is a multiple of 4: 8-byte aligned
-                    ah[i] = *reinterpret_cast<s16x4 const *>(&s.ah[ar][kq]);
-                    al[i] = *reinterpret_cast<s16x4 const *>(&s.al[ar][kq]);
-                    bh[i] = *reinterpret_cast<s16x4 const *>(&s.bh[bc][kq]);
-                    bl[i] = *reinterpret_cast<s16x4 const *>(&s.bl[bc][kq]);
+                    // rows are 80 bytes and kq is a multiple of 8: 16-byte aligned
+                    ah[i] = *reinterpret_cast<bf16x8 const *>(&s.ah[ar][kq]);
+                    al[i] = *reinterpret_cast<bf16x8 const *>(&s.al[ar][kq]);
+                    bh[i] = *reinterpret_cast<bf16x8 const *>(&s.bh[bc][kq]);
+                    bl[i] = *reinterpret_cast<bf16x8 const *>(&s.bl[bc][kq]);
                 }
 #pragma unroll
                 for (int i = 0; i < 2; ++i)
 #pragma unroll
                     for (int j = 0; j < 2; ++j)
                     {
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[i], bh[j], acc[i][j], 0, 0, 0);
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bl[j], acc[i][j], 0, 0, 0);
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bh[j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
                     }
             }
         }

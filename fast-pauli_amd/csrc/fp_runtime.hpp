@@ -24,8 +24,33 @@ namespace fprt
 fp_status fail(fp_status code, char const *fmt, ...) __attribute__((format(printf, 2, 3)));
 
 // Per-stream device scratch of at least `bytes`; the pointer stays valid until
-// the next, larger request on the same stream.
+// the next, larger request on the same stream.  Callers hold a StreamGuard for
+// the stream until the kernels that use the buffer are enqueued.
 fp_status scratch_get(hipStream_t stream, size_t bytes, void **out);
+
+// Serialises the library's entry points on one (device, stream): an entry point
+// that uses the stream's scratch holds the guard until its kernels are enqueued,
+// so that calls from several host threads on one stream neither interleave
+// their kernels nor free a buffer another call is about to use.
+class StreamGuard
+{
+  public:
+    explicit StreamGuard(hipStream_t stream);
+    ~StreamGuard();
+    StreamGuard(StreamGuard const &) = delete;
+    StreamGuard &operator=(StreamGuard const &) = delete;
+
+  private:
+    std::recursive_mutex *mu_;
+};
+
+// Device workspace supplied by the caller instead of the stream's scratch (the
+// allocation-free entry point passes its static buffer).
+struct Workspace
+{
+    void *base;
+    size_t bytes;
+};
 
 // A launch stays below 2^32 work-items: at most 2^23 blocks of 256 threads.
 constexpr uint64_t kMaxLaunchBlocks = uint64_t(1) << 23;
@@ -95,8 +120,18 @@ fp_status choose_op_expect_tiled(fp_hip_ops const *ops, int dtype, size_t n_stat
 fp_status choose_op_apply_tiled(fp_hip_ops const *ops, int dtype, size_t n_states, bool *use_tiled);
 fp_status run_op_apply_tiled(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim,
                              size_t n_states, int accumulate, hipStream_t st);
+// With `ws`, the string-group tables and the partials go into the workspace (no
+// allocation; fails with FP_ERR_INVALID_ARGUMENT when they cannot fit), else the
+// tables' device copy and the stream's scratch.
 fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim,
-                              size_t n_states, int accumulate, hipStream_t st);
+                              size_t n_states, int accumulate, hipStream_t st, Workspace const *ws = nullptr);
+// The streaming PauliOp expectation (strings [s_begin, s_begin + count), per-string
+// complex scales, out[t] (+)= post * sum), partials in `ws` or the stream's scratch.
+fp_status run_op_expect(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim, size_t n_states,
+                        size_t s_begin, size_t count, double const *scale_table, double post_re, double post_im,
+                        int accumulate, hipStream_t st, Workspace const *ws = nullptr);
+// Shape checks shared by the entry points.
+fp_status check_shape(fp_hip_ops const *ops, int dtype, void const *a, void const *b, size_t dim, size_t n_states);
 
 } // namespace fprt
 
@@ -163,4 +198,7 @@ struct fp_hip_ops
     // string groups of the tiled expectation kernel, built on first use per geometry
     mutable std::mutex tiles_mu;
     mutable std::list<fprt::TileTables> tiles;
+    // bumped whenever strings, coefficients or the matrix change, so that
+    // batches built over this set can tell that their tables are stale
+    uint64_t generation = 0;
 };

@@ -40,6 +40,10 @@ double now_s()
     return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
 
+// Largest state or output buffer of one request (2^50 bytes): far beyond any
+// device, and small enough that size classes and budget sums cannot overflow.
+constexpr size_t kMaxRequestBytes = size_t(1) << 50;
+
 struct Block
 {
     void *ptr = nullptr;
@@ -54,10 +58,12 @@ class BufferPool
     {
     }
 
+    // The power-of-two class holding `bytes` (at least 4 KiB); requests are
+    // limited to kMaxRequestBytes at submission, so the class never overflows.
     static size_t size_class(size_t bytes)
     {
         size_t c = 4096;
-        while (c < bytes)
+        while (c < bytes && c <= (SIZE_MAX >> 1))
             c <<= 1;
         return c;
     }
@@ -501,6 +507,20 @@ fp_status fp_hip_server_submit(fp_hip_server *s, fp_serve_request const *req, ui
         return fail(FP_ERR_INVALID_ARGUMENT, "unknown dtype %d", req->dtype);
     if (req->n_qubits == 0 || req->n_qubits > 40 || req->n_strings == 0 || req->n_states == 0 || !req->codes)
         return fail(FP_ERR_INVALID_ARGUMENT, "empty or oversized request");
+    if (req->n_strings > 0xffffffffu)
+        return fail(FP_ERR_INVALID_ARGUMENT, "too many strings (%zu)", req->n_strings);
+    {
+        // state and output bytes must be representable, and below the largest pool class
+        size_t const esz = elem_bytes(req->dtype);
+        size_t const rows_out = req->kind == FP_BATCH_APPLY ? (size_t(1) << req->n_qubits) : 1;
+        size_t in_b = 0, out_b = 0;
+        if (__builtin_mul_overflow(size_t(1) << req->n_qubits, req->n_states, &in_b) ||
+            __builtin_mul_overflow(in_b, esz, &in_b) || __builtin_mul_overflow(rows_out, req->n_states, &out_b) ||
+            __builtin_mul_overflow(out_b, esz, &out_b) || in_b > kMaxRequestBytes || out_b > kMaxRequestBytes)
+            return fail(FP_ERR_INVALID_ARGUMENT,
+                        "request of %zu qubits x %zu states is too large to address (state or output above %zu bytes)",
+                        req->n_qubits, req->n_states, kMaxRequestBytes);
+    }
     for (size_t i = 0; i < req->n_qubits * req->n_strings; ++i)
         if (req->codes[i] > 3)
             return fail(FP_ERR_INVALID_ARGUMENT, "Pauli code must be 0, 1, 2, or 3");

@@ -535,6 +535,9 @@ fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, 
     {
         ops->tiles.emplace_back();
         t = &ops->tiles.back();
+        hipError_t const e = hipEventCreateWithFlags(&t->uploaded, hipEventDisableTiming);
+        if (e != hipSuccess)
+            return fail(FP_ERR_RUNTIME, "tile table event creation failed: %s", hipGetErrorString(e));
     }
     if (t->uploaded)
     {
@@ -670,15 +673,45 @@ int ev_group_mode(fpt::TileGroup const &g, int dtype, TileGeom const &geo)
 }
 
 fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim,
-                              size_t n_states, int accumulate, hipStream_t st)
+                              size_t n_states, int accumulate, hipStream_t st, Workspace const *ws)
 {
     TileGeom const geo = tile_geom(dtype, ops->n_qubits, n_states, true);
     TileTables *tt = nullptr;
     fp_status s = tile_tables(ops, kTilesOpExpect, geo.k, geo.dh, &tt);
     if (s != FP_OK)
         return s;
-    if ((s = tile_device(ops, tt, st)) != FP_OK)
-        return s;
+    // device addresses of the tables: their own device copy, or the workspace's head
+    fpt::TileGroup const *d_groups;
+    uint64_t const *d_btab;
+    void const *d_chunks, *d_strs;
+    fpt::ShStr const *d_sh;
+    size_t const table_bytes = (tt->h.size() + 255) / 256 * 256;
+    if (ws)
+    {
+        if (table_bytes > ws->bytes)
+            return fail(FP_ERR_INVALID_ARGUMENT, "the string-group tables (%zu bytes) exceed the %zu-byte workspace",
+                        table_bytes, ws->bytes);
+        std::lock_guard<std::mutex> lock(ops->tiles_mu);
+        FP_HIP(hipMemcpyAsync(ws->base, tt->h.data(), tt->h.size(), hipMemcpyHostToDevice, st));
+        // the host image may be rebuilt only once this copy has read it
+        FP_HIP(hipEventRecord(tt->uploaded, st));
+        auto *base = static_cast<unsigned char *>(ws->base);
+        d_groups = reinterpret_cast<fpt::TileGroup const *>(base);
+        d_btab = reinterpret_cast<uint64_t const *>(base + tt->o_btab);
+        d_chunks = base + tt->o_chunks;
+        d_strs = base + tt->o_strs;
+        d_sh = reinterpret_cast<fpt::ShStr const *>(base + tt->o_sh);
+    }
+    else
+    {
+        if ((s = tile_device(ops, tt, st)) != FP_OK)
+            return s;
+        d_groups = tt->groups;
+        d_btab = tt->btab;
+        d_chunks = tt->chunks;
+        d_strs = tt->strs;
+        d_sh = tt->sh;
+    }
     uint64_t const n_cosets = uint64_t(dim) >> geo.d;
     uint32_t const chunks_y = uint32_t((n_states + (size_t(1) << geo.lsb) - 1) >> geo.lsb);
     if (chunks_y > 65535u)
@@ -689,12 +722,27 @@ fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *stat
     uint64_t const target = std::max<uint64_t>(256, std::min<uint64_t>(env_u32("FAST_PAULI_TILE_BLOCKS", 16384), tiles / 16));
     uint64_t want = (target + uint64_t(tt->n_groups) * chunks_y - 1) / (uint64_t(tt->n_groups) * chunks_y);
     want = std::max<uint64_t>(1, std::min<uint64_t>(want, n_cosets));
+    uint64_t const part_bytes = uint64_t(n_states) * sizeof(fpk::Dbl2); // per (group, range)
+    if (ws)
+    {
+        // fewer blocks per group when the workspace cannot hold their partials
+        uint64_t const room = (ws->bytes - table_bytes) / part_bytes;
+        if (room < tt->n_groups)
+            return fail(FP_ERR_INVALID_ARGUMENT,
+                        "%u string groups x %zu states need %llu bytes of partials beside %zu bytes of tables, "
+                        "more than the %zu-byte workspace",
+                        tt->n_groups, n_states, static_cast<unsigned long long>(uint64_t(tt->n_groups) * part_bytes),
+                        table_bytes, ws->bytes);
+        want = std::min<uint64_t>(want, room / tt->n_groups);
+    }
     uint32_t const ranges = uint32_t(std::min<uint64_t>(want, 0x7fffffffu));
     uint64_t const n_parts = uint64_t(tt->n_groups) * ranges;
     if (n_parts > 0x7fffffffu)
         return fail(FP_ERR_INVALID_ARGUMENT, "too many tile groups");
     void *scratch = nullptr;
-    if ((s = scratch_get(st, n_parts * n_states * sizeof(fpk::Dbl2), &scratch)) != FP_OK)
+    if (ws)
+        scratch = static_cast<unsigned char *>(ws->base) + table_bytes;
+    else if ((s = scratch_get(st, n_parts * part_bytes, &scratch)) != FP_OK)
         return s;
     auto *partial = static_cast<fpk::Dbl2 *>(scratch);
     size_t const lds = size_t(geo.tile_kb) * 1024u;
@@ -759,9 +807,9 @@ fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *stat
                              : mode == 1 ? &fpt::k_op_expect_tiled<T, WIDE, TKB, 1>
                                          : &fpt::k_op_expect_tiled<T, WIDE, TKB, 0>;
             hipLaunchKernelGGL(kfn, grid, dim3(TKB * 8), lds, st, typed_states, uint64_t(n_states), sshift, geo.lsb,
-                               geo.k, geo.d, ranges, uint32_t(n_parts), chunks_y, n_cosets, tt->groups, tt->btab,
-                               static_cast<fpt::TileChunk const *>(tt->chunks),
-                               static_cast<fpt::TileStr const *>(tt->strs), tt->sh, partial);
+                               geo.k, geo.d, ranges, uint32_t(n_parts), chunks_y, n_cosets, d_groups, d_btab,
+                               static_cast<fpt::TileChunk const *>(d_chunks),
+                               static_cast<fpt::TileStr const *>(d_strs), d_sh, partial);
             FP_LAUNCHED();
         }
         hipLaunchKernelGGL(fpk::k_reduce_partials<T>, dim3(uint32_t(n_states)), dim3(fpk::kBlock), 0, st, partial,

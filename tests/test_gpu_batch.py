@@ -173,3 +173,45 @@ def test_uneven_workload_autotuned_bit_exact(hip, tmp_path):
     # cached schedule is reused
     again = b.autotune(reps=1, cache_path=str(tmp_path / "sched.json"))
     assert again["cached"] and again["schedule"] == res["schedule"]
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_rebuilds_after_ops_reassigned(hip, graph):
+    """A batch built over an operator set must not replay stale tables after the
+    set is re-targeted to more strings (its device arena is reallocated) or given
+    new coefficients: the next launch rebuilds and matches the operator alone."""
+    H.say(f"batch after ops.assign graph={graph}")
+    n, S = 8, 5
+    psi = syn.random_states(1 << n, S, seed=3).astype(np.complex64)
+    d_in = _abi.DeviceBuffer.from_array(psi)
+    d_out = _abi.DeviceBuffer(psi.nbytes)
+    d_ev = _abi.DeviceBuffer(S * 8)
+    ops = _abi.Ops(syn.random_codes(16, n, seed=1), syn.random_coeffs(16, seed=1))
+    b = Batch()
+    if not graph:
+        _abi.call("fp_hip_batch_set_graph", b.handle, 0)
+    b.add_apply(ops, d_in.ptr, d_out.ptr, 1 << n, S, C64)
+    b.add_expectation(ops, d_in.ptr, d_ev.ptr, 1 << n, S, C64)
+    b.launch()
+    _abi.synchronize()
+    assert np.array_equal(d_out.to_array(np.complex64, psi.shape), H.op_apply(ops, psi, C64))
+    # grow the set well beyond its arena, then launch the same batch again
+    codes2, coeffs2 = syn.random_codes(300, n, seed=2), syn.random_coeffs(300, seed=2)
+    ops.assign(codes2, coeffs2)
+    b.launch()
+    _abi.synchronize()
+    fresh = _abi.Ops(codes2, coeffs2)
+    assert np.array_equal(d_out.to_array(np.complex64, psi.shape), H.op_apply(fresh, psi, C64))
+    assert H.rel_err(d_ev.to_array(np.complex64, (S,)), H.op_expectation(fresh, psi, C64)) <= 1e-5
+    # new coefficients only
+    coeffs3 = syn.random_coeffs(300, seed=3)
+    ops.set_coeffs(coeffs3)
+    b.launch()
+    _abi.synchronize()
+    fresh = _abi.Ops(codes2, coeffs3)
+    assert np.array_equal(d_out.to_array(np.complex64, psi.shape), H.op_apply(fresh, psi, C64))
+    # a set re-targeted to another qubit count no longer fits the entry: a clear error
+    ops.assign(syn.random_codes(4, n + 1, seed=4))
+    with pytest.raises(ValueError):
+        b.launch()
+    b.destroy()

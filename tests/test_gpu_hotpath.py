@@ -516,6 +516,14 @@ def test_shape_errors(hip):
         _abi.call("fp_hip_string_apply_batch", ops.handle, 5, C128, d.ptr, d.ptr, 8, 1, 1.0, 0.0, 0, None)
     with pytest.raises(ValueError):
         _abi.Ops(np.array([[0, 4]], np.uint8))
+    # partially overlapping input and output (out starts one element into states)
+    d2 = _abi.DeviceBuffer(16 * 9)
+    with pytest.raises(ValueError):
+        _abi.call("fp_hip_string_apply_batch", ops.handle, 0, C128, d2.ptr, d2.ptr + 16, 8, 1, 1.0, 0.0, 0, None)
+    with pytest.raises(ValueError):
+        _abi.call("fp_hip_op_apply", ops.handle, C128, d2.ptr + 16, d2.ptr, 8, 1, 0, None)
+    with pytest.raises(ValueError):
+        _abi.call("fp_hip_op_apply", ops.handle, C64, d2.ptr, d2.ptr + 56, 8, 1, 0, None)
 
 
 @pytest.mark.parametrize("dtype,n,S,kernel", [

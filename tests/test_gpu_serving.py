@@ -95,3 +95,20 @@ def test_poisson_run_reports_percentiles(hip):
     assert out["failed"] == 0
     assert 0 < out["p50_ms"] <= out["p99_ms"]
     print(out)
+
+
+def test_oversized_request_rejected_at_submit(hip):
+    """State or output bytes that do not fit in size_t (2^40 rows x 2^20 states x
+    16 B = 2^64) are refused at submission with a clear error; nothing is queued
+    and the loop keeps serving."""
+    H.say("serving oversized request at submit")
+    with Server(hbm_budget_bytes=1 << 30, n_slots=1, max_batch=4) as srv:
+        codes = np.zeros((1, 40), np.uint8)
+        for n_states in (1 << 20, (1 << 24) + 3):
+            with pytest.raises(ValueError):
+                srv.submit(_abi.FP_BATCH_EXPECT, codes, None, n_states, 1, C128)
+        with pytest.raises(ValueError):
+            srv.submit(_abi.FP_BATCH_APPLY, codes, None, 1 << 20, 1, C128)
+        rid = srv.submit_expectation(syn.random_codes(4, 8, seed=1), n_states=2, seed=1)
+        assert srv.wait(rid, 60)["status"] == 0
+        assert srv.stats()["submitted"] == 1
