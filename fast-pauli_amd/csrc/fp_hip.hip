@@ -819,7 +819,7 @@ fp_status fp_hip_init(int device)
         return fail(FP_ERR_INVALID_ARGUMENT, "device %d out of range (%d devices)", device, n);
     FP_HIP(hipSetDevice(device));
     FP_HIP(hipFree(nullptr)); // force context creation
-    return FP_OK;
+    return noalloc_setup();
 }
 
 fp_status fp_hip_malloc(void **ptr, size_t bytes)

@@ -132,6 +132,9 @@ fp_status run_op_expect(fp_hip_ops const *ops, int dtype, void const *states, vo
                         int accumulate, hipStream_t st, Workspace const *ws = nullptr);
 // Shape checks shared by the entry points.
 fp_status check_shape(fp_hip_ops const *ops, int dtype, void const *a, void const *b, size_t dim, size_t n_states);
+// Sets up the allocation-free entry point's static buffer on the current device
+// (called by fp_hip_init; idempotent).
+fp_status noalloc_setup();
 
 } // namespace fprt
 

@@ -87,6 +87,7 @@ _SIGS = {
     "fp_hip_op_expectation": [_vp, _i, _vp, _vp, _sz, _sz, _i, _vp],
     "fp_hip_op_expectation_plan": [_vp, _i, _sz, _sz, _vp],
     "fp_hip_op_apply_plan": [_vp, _i, _sz, _sz, _i, _vp],
+    "fp_hip_op_expectation_noalloc": [_vp, _i, _vp, _vp, _sz, _sz, _i, _vp],
     "fp_hip_strings_expectation_matrix": [_vp, _i, _vp, _vp, _sz, _sz, _vp],
     "fp_hip_summed_expectation": [_vp, _i, _i, _vp, _vp, _sz, _sz, _i, _vp],
     "fp_hip_summed_apply": [_vp, _i, _vp, _vp, _sz, _sz, _i, _vp],

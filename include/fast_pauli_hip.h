@@ -165,6 +165,24 @@ fp_status fp_hip_op_apply(fp_hip_ops const *ops, int dtype, void const *states, 
 fp_status fp_hip_op_expectation(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim,
                                 size_t n_states, int accumulate, void *stream);
 
+/* Size of the static device workspace of fp_hip_op_expectation_noalloc (per GPU). */
+#define FP_NOALLOC_SCRATCH_BYTES ((size_t)32 << 20)
+
+/* fp_hip_op_expectation without any device allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same kernels, same results as fp_hip_op_expectation (identical bits
+ * whenever the partials fit beside the tables without fewer blocks; otherwise a regrouped FP64
+ * sum); the reduction partials and the string-group tables live in one static
+ * FP_NOALLOC_SCRATCH_BYTES device array per GPU instead of library-allocated memory.  Shapes
+ * whose tables and partials (tiled: groups x n_states x 16 B at least; streaming:
+ * n_states x 16 B at least) exceed it fail with FP_ERR_INVALID_ARGUMENT before any kernel runs.
+ * The first call for an operator and a (dtype, n_states) geometry builds the host-side string
+ * groups, as fp_hip_op_expectation does, and caches them in the operator; the tables are copied
+ * into the workspace on `stream` by every call.  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (on different streams, or from several host threads) race. */
+fp_status fp_hip_op_expectation_noalloc(fp_hip_ops const *ops, int dtype, void const *states, void *out,
+                                        size_t dim, size_t n_states, int accumulate, void *stream);
+
 /* How fp_hip_op_expectation evaluates these shapes.  No counterpart in the reference:
  * diagnostics and roofline accounting for callers that measure the kernel.
  *   kernel 0 (FP_EV_STREAM): every string reads the whole state once.
