@@ -78,6 +78,11 @@ def parse():
     ap.add_argument("--pmc-pass", action="store_true",
                     help="only run the profiled kernels once each (a plain copy of the state, apply_batch, "
                          "expectation_value) for hardware-counter runs")
+    ap.add_argument("--pmc-summed", action="store_true",
+                    help="only run SummedPauliOp.expectation_value at n=14 (2000 strings, 200 operators, 1000 "
+                         "states) once in complex128/FP64 and once in complex64/FP32, for profiler runs")
+    ap.add_argument("--pmc-batch", action="store_true",
+                    help="only build the 256-operator uneven batch and launch it twice, for profiler runs")
     ap.add_argument("--pmc-gemm", action="store_true",
                     help="only run the SummedPauliOp contraction (200 x 2000) x (2000 x 1000) once per MFMA mode "
                          "(FP64, FP32, BF16x3) for hardware-counter runs")
@@ -102,6 +107,52 @@ def pmc_gemm() -> int:
     return 0
 
 
+def pmc_summed(args) -> int:
+    """The n=14 SummedPauliOp workload, once per state precision (stage 1 + contraction)."""
+    from fast_pauli import _abi
+    from fast_pauli import synthetic as syn
+
+    _abi.init(0)
+    n, N, K, S = 14, 2000, 200, 1000
+    dim = 1 << n
+    codes = syn.random_codes_max_weight(N, n, 3, seed=args.seed)
+    W = syn.random_real_matrix(K, N, seed=args.seed)
+    ops = _abi.Ops(codes, matrix=W.T.astype(np.complex128))
+    d128 = _abi.DeviceBuffer(dim * S * 16)
+    d64 = _abi.DeviceBuffer(dim * S * 8)
+    _abi.call("fp_hip_fill_states", _abi.FP_C128, d128.ptr, dim, S, syn.gpu_seed(args.seed), 1, None)
+    _abi.call("fp_hip_convert", _abi.FP_C128, d128.ptr, _abi.FP_C64, d64.ptr, dim * S, None)
+    out = _abi.DeviceBuffer(K * S * 16)
+    for dt, mode, src in ((_abi.FP_C128, _abi.FP_CONTRACT_FP64, d128), (_abi.FP_C64, _abi.FP_CONTRACT_FP32, d64)):
+        _abi.call("fp_hip_summed_expectation", ops.handle, dt, mode, src.ptr, out.ptr, dim, S, 0, None)
+        _abi.synchronize()
+    log("pmc summed pass done: n=14, 2000 strings, 200 operators, 1000 states, complex128/FP64 then complex64/FP32")
+    return 0
+
+
+def pmc_batch(args) -> int:
+    """The 256-operator uneven batch, default schedule: one build launch, one replay."""
+    from fast_pauli import _abi, synthetic as syn
+    from fast_pauli.batch import Batch, uneven_workload
+
+    _abi.init(0)
+    b = Batch()
+    keep = []
+    for sp in uneven_workload(256, seed=args.seed):
+        n, N, S = sp["n"], sp["strings"], sp["states"]
+        ops = _abi.Ops(syn.random_codes(N, n, seed=sp["seed"]), syn.random_coeffs(N, seed=sp["seed"]))
+        d_in = _abi.DeviceBuffer((1 << n) * S * 8)
+        d_out = _abi.DeviceBuffer((1 << n) * S * 8)
+        _abi.call("fp_hip_fill_states", _abi.FP_C64, d_in.ptr, 1 << n, S, sp["seed"], 1, None)
+        b.add_apply(ops, d_in.ptr, d_out.ptr, 1 << n, S, _abi.FP_C64)
+        keep.append((ops, d_in, d_out))
+    b.launch()
+    b.launch()
+    _abi.synchronize()
+    log(f"pmc batch pass done: {len(keep)} operators, {b.info()}")
+    return 0
+
+
 def main() -> int:
     args = parse()
     # the CPU baseline's OpenMP team: every available core unless the caller says otherwise,
@@ -112,6 +163,10 @@ def main() -> int:
     os.environ.setdefault("OMP_PROC_BIND", "close")
     if args.pmc_gemm:
         return pmc_gemm()
+    if args.pmc_summed:
+        return pmc_summed(args)
+    if args.pmc_batch:
+        return pmc_batch(args)
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only; sharding over several GPUs "
                           "is not implemented", "n_gpus": args.gpus}), flush=True)
@@ -351,9 +406,9 @@ def main() -> int:
 def measured_traffic(kernel: str, n: int, S: int, n_strings: int):
     """HBM bytes per call of `kernel` (summed over its launches when one call makes
     several, as the tiled PauliOp.apply does per string group) from the committed counter summary
-    (profiles/pmc_traffic_n24_s16.json: FETCH_SIZE and WRITE_SIZE, each scaled by
-    the plain copy kernel's reading for a known byte count), for the shape it
-    was measured at; None otherwise."""
+    (profiles/pmc_traffic_n24_s16.json: read bytes from the TCC-to-memory read requests by size,
+    32/64/128 B, write bytes from the 32/64-byte write requests, no scaling factor), for the
+    shape it was measured at; None otherwise."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic_n24_s16.json")
     try:
         with open(path) as f:

@@ -100,8 +100,9 @@ template <class T> __device__ __forceinline__ T add_exact(T a, T b)
 
 // ---------------------------------------------------------------------------
 // K1: single-string apply.  out(j,:) (+)= cf * (-1)^{popcount(j&z)} * in(j^x,:)
-// Each thread moves U vectors spaced one block apart; reads of the partner
-// row j^x stay coalesced because the row is contiguous along the states axis.
+// Each thread moves U vectors spaced one block apart (the launch uses U = 1);
+// reads of the partner row j^x stay coalesced because the row is contiguous
+// along the states axis.
 // ---------------------------------------------------------------------------
 template <class T, int NC, int U, bool POW2, bool ACC>
 __global__ __launch_bounds__(kBlock) void k_string_apply(typename VecOf<T, NC>::type const *__restrict__ in,

@@ -52,6 +52,16 @@ struct Workspace
     size_t bytes;
 };
 
+// LDS bytes one workgroup may use on the current device (160 KiB on gfx950), from
+// the device's attributes, cached per device.
+size_t lds_bytes();
+// LDS bytes of one state column in the LDS-resident expectation kernels: the
+// workgroup's LDS less room for their static counters.
+inline size_t column_lds_bytes()
+{
+    return lds_bytes() - 256;
+}
+
 // A launch stays below 2^32 work-items: at most 2^23 blocks of 256 threads.
 constexpr uint64_t kMaxLaunchBlocks = uint64_t(1) << 23;
 

@@ -635,6 +635,23 @@ void release_tiles(fp_hip_ops *ops)
     ops->tiles.clear();
 }
 
+// The string-group tables `tt` were cut for `geo`, and one tile of that geometry
+// (2^d rows x 2^lsb states) fits the LDS tile of the kernel instance about to run.
+fp_status check_tile_fit(TileTables const *tt, TileGeom const &geo, int dtype, size_t tile_bytes)
+{
+    size_t const esz = dtype == FP_C128 ? 16 : 8;
+    if (tt->k != geo.k || tt->dh != geo.dh)
+        return fail(FP_ERR_RUNTIME, "string-group tables built for k=%u dh=%u, launch geometry k=%u dh=%u", tt->k,
+                    tt->dh, geo.k, geo.dh);
+    if (geo.d != geo.k + geo.dh || (size_t(1) << (geo.d + geo.lsb)) * esz > tile_bytes)
+        return fail(FP_ERR_RUNTIME, "a tile of 2^%u rows x 2^%u states does not fit the kernel's %zu-byte LDS tile",
+                    geo.d, geo.lsb, tile_bytes);
+    if (tile_bytes > lds_bytes())
+        return fail(FP_ERR_UNSUPPORTED, "%zu-byte LDS tiles exceed the device's %zu bytes per workgroup", tile_bytes,
+                    lds_bytes());
+    return FP_OK;
+}
+
 // 0: streaming kernel, 1: tiled kernel, per FAST_PAULI_EV_KERNEL ("stream",
 // "tiled") or, by default, by estimated HBM traffic (groups vs strings).
 fp_status choose_op_expect_tiled(fp_hip_ops const *ops, int dtype, size_t n_states, bool *use_tiled)
@@ -679,6 +696,8 @@ fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *stat
     TileTables *tt = nullptr;
     fp_status s = tile_tables(ops, kTilesOpExpect, geo.k, geo.dh, &tt);
     if (s != FP_OK)
+        return s;
+    if ((s = check_tile_fit(tt, geo, dtype, size_t(geo.tile_kb) << 10)) != FP_OK)
         return s;
     // device addresses of the tables: their own device copy, or the workspace's head
     fpt::TileGroup const *d_groups;
@@ -865,6 +884,8 @@ fp_status run_op_apply_tiled(fp_hip_ops const *ops, int dtype, void const *state
     fp_status s = tile_tables(ops, kTilesOpApply, geo.k, geo.dh, &tt);
     if (s != FP_OK)
         return s;
+    if ((s = check_tile_fit(tt, geo, dtype, fpa::kTileBytes)) != FP_OK)
+        return s;
     if ((s = tile_device(ops, tt, st)) != FP_OK)
         return s;
     uint64_t const n_cosets = uint64_t(dim) >> geo.d;
@@ -941,13 +962,11 @@ struct CosetGeom
     uint32_t d, k, n_cos;
 };
 
-constexpr size_t kColumnLds = 160 * 1024 - 256;
-
 CosetGeom coset_geom(int dtype, size_t n_qubits)
 {
     size_t const esz = dtype == FP_C128 ? 16 : 8;
     CosetGeom g{};
-    g.d = log2u(kColumnLds / esz);             // 13 (complex128) or 14 (complex64)
+    g.d = log2u(column_lds_bytes() / esz);     // 13 (complex128) or 14 (complex64) with 160 KiB
     g.k = dtype == FP_C128 ? 3u : 4u;          // 128-byte runs of consecutive rows
     g.n_cos = n_qubits > g.d && n_qubits - g.d < 31 ? 1u << (n_qubits - g.d) : 0u;
     return g;
@@ -995,7 +1014,7 @@ fp_status launch_ev_coset(fp_hip_ops const *ops, TileTables *tt, CosetGeom const
     static hipError_t attr = hipSuccess;
     std::call_once(once, [] {
         attr = hipFuncSetAttribute(reinterpret_cast<void const *>(&fpl::k_ev_coset<T>),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(kColumnLds));
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(column_lds_bytes()));
     });
     if (attr != hipSuccess)
         return fail(FP_ERR_RUNTIME, "hipFuncSetAttribute failed: %s", hipGetErrorString(attr));
