@@ -224,64 +224,28 @@ template <std::floating_point T> struct SummedPauliOp
     }
 
     // A_k^2 = sum_{i,j} h_ik h_jk P_i P_j, collected over all strings of weight
-    // <= min(n, 2 * max weight) in calculate_pauli_strings_max_weight order.
+    // <= min(n, 2 * max weight) in calculate_pauli_strings_max_weight order
+    // (__summed_pauli_op.hpp:197-268).  The pair products and the contraction run on
+    // the device (fp_hip_summed_square), IEEE-equal to the reference's sequential loop.
     SummedPauliOp square() const
     {
         std::size_t max_w = 0;
         for (auto const &p : pauli_strings)
             max_w = std::max<std::size_t>(max_w, p.weight);
         std::vector<PauliString> sq = calculate_pauli_strings_max_weight(n_qubits(), std::min(n_qubits(), 2 * max_w));
-        std::size_t const K = n_operators(), N = n_pauli_strings(), n = n_qubits();
-        std::vector<std::complex<T>> out(sq.size() * K);
-        auto accumulate = [&](std::size_t a, std::complex<double> phase, std::size_t i, std::size_t j) {
-            std::complex<T> const ph(phase);
-            for (std::size_t k = 0; k < K; ++k)
-                out[a * K + k] += ph * coeffs(i, k) * coeffs(j, k);
-        };
-        if (n <= 32)
+        std::size_t const K = n_operators();
+        std::size_t count = 0;
+        hip::check(fp_hip_summed_square(device_ops(), hip::dtype_of<T>(), nullptr, &count, nullptr));
+        if (count != sq.size())
+            throw std::runtime_error("square: the device and the host enumerate different string counts");
+        std::vector<std::complex<T>> out(count * K);
+        if (!out.empty())
         {
-            // product strings found by their (x, z) bit masks; the phase is the same
-            // letter-by-letter product as operator*, so results are unchanged
-            auto key_of = [n](std::vector<Pauli> const &ps) {
-                std::uint64_t x = 0, z = 0;
-                for (std::size_t q = 0; q < n; ++q)
-                {
-                    x |= std::uint64_t(ps[q].code == 1 || ps[q].code == 2) << q;
-                    z |= std::uint64_t(ps[q].code == 2 || ps[q].code == 3) << q;
-                }
-                return (x << 32) | z;
-            };
-            std::unordered_map<std::uint64_t, std::size_t> index;
-            index.reserve(sq.size());
-            for (std::size_t a = 0; a < sq.size(); ++a)
-                index.emplace(key_of(sq[a].paulis), a);
-            std::vector<Pauli> prod(n);
-            for (std::size_t i = 0; i < N; ++i)
-                for (std::size_t j = 0; j < N; ++j)
-                {
-                    std::complex<double> phase = 1;
-                    for (std::size_t q = 0; q < n; ++q)
-                    {
-                        auto [ph, p] = pauli_strings[i].paulis[q] * pauli_strings[j].paulis[q];
-                        phase *= ph;
-                        prod[q] = p;
-                    }
-                    accumulate(index.at(key_of(prod)), phase, i, j);
-                }
+            hip::Buffer d(out.size() * sizeof(std::complex<T>));
+            hip::check(fp_hip_summed_square(device_ops(), hip::dtype_of<T>(), d.get(), &count, nullptr));
+            hip::d2h(out.data(), d.get(), out.size() * sizeof(std::complex<T>));
         }
-        else
-        {
-            std::map<std::string, std::size_t> index;
-            for (std::size_t a = 0; a < sq.size(); ++a)
-                index.emplace(sq[a].str(), a);
-            for (std::size_t i = 0; i < N; ++i)
-                for (std::size_t j = 0; j < N; ++j)
-                {
-                    auto [phase, prod] = pauli_strings[i] * pauli_strings[j];
-                    accumulate(index.at(prod.str()), phase, i, j);
-                }
-        }
-        return SummedPauliOp(sq, Tensor<2>(out.data(), sq.size(), K));
+        return SummedPauliOp(sq, Tensor<2>(out.data(), count, K));
     }
 
     // Dense matrices (__summed_pauli_op.hpp:644-665), accumulated: on the host, or

@@ -93,6 +93,7 @@ _SIGS = {
     "fp_hip_summed_apply": [_vp, _i, _vp, _vp, _sz, _sz, _i, _vp],
     "fp_hip_summed_apply_weighted": [_vp, _i, _vp, _vp, _vp, _sz, _sz, _i, _vp],
     "fp_hip_gemm": [_i, _sz, _sz, _sz, _vp, _vp, _vp, _vp],
+    "fp_hip_summed_square": [_vp, _i, _vp, ctypes.POINTER(_sz), _vp],
     "fp_hip_string_sparse_repr": [_vp, _sz, _i, _vp, _vp, _sz, _vp],
     "fp_hip_string_to_tensor": [_vp, _sz, _i, _vp, _sz, _vp],
     "fp_hip_op_to_tensor": [_vp, _i, _vp, _sz, _i, _vp],

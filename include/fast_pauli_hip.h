@@ -245,6 +245,17 @@ fp_status fp_hip_summed_apply(fp_hip_ops const *ops, int dtype, void const *stat
 fp_status fp_hip_summed_apply_weighted(fp_hip_ops const *ops, int dtype, void const *states, void const *data,
                                        void *out, size_t dim, size_t n_states, int accumulate, void *stream);
 
+/* SummedPauliOp::square (fast_pauli/cpp/include/__summed_pauli_op.hpp:197-268): the operators'
+ * squares A_k^2 = sum_a h'(a,k) P_a over the strings of weight <= W = min(n, 2 * the set's largest
+ * weight), in the order of calculate_pauli_strings_max_weight(n, W)
+ * (fast_pauli/cpp/include/__pauli_helpers.hpp:147-153).  *n_out_strings receives their count;
+ * with out = NULL nothing else happens (a size query).  out: (n_out_strings x n_operators) complex of
+ * `dtype` on the device, overwritten.  Each h'(a,k) sums phase_ij h(i,k) h(j,k) over the pairs with
+ * P_i P_j = phase_ij P_a in (i, j) order, as (phase * h(i,k)) * h(j,k), the reference's operations in
+ * its order, so results are IEEE-equal to its sequential loop.  Limits: n_strings^2 < 2^31 and
+ * n_out_strings < 2^32 (FP_ERR_UNSUPPORTED beyond). */
+fp_status fp_hip_summed_square(fp_hip_ops const *ops, int dtype, void *out, size_t *n_out_strings, void *stream);
+
 /* ---- dense and sparse materialisation on the device (debugging, interop) --------
  * Values are IEEE-equal to the reference's: m(j) = (-i)^{nY} (-1)^{popcount(j & z)} formed
  * as the reference forms it, c * m as std::complex multiplies, strings added in order. */

@@ -17,6 +17,10 @@
 //   * SummedPauliOp::apply        __summed_pauli_op.hpp:287-349
 //   * SummedPauliOp::apply_weighted __summed_pauli_op.hpp:378-503
 //   * SummedPauliOp::expectation_value __summed_pauli_op.hpp:530-614
+//   * Pauli / PauliString products  __pauli.hpp:110-178, __pauli_string.hpp:228-247
+//   * calculate_pauli_strings_max_weight  __pauli_helpers.hpp:33-153
+//   * SummedPauliOp::square         __summed_pauli_op.hpp:197-268
+//   * PauliOp * PauliOp             __pauli_op.hpp:237-281 (one thread)
 // with the same schedules, per-thread scratch and reduction order, so that it
 // doubles as the reference's CPU baseline.  `par` selects the OpenMP branch as
 // is_parallel_execution_policy_v does (__type_traits.hpp:51-53).
@@ -31,10 +35,13 @@
 
 #include <omp.h>
 
+#include <algorithm>
 #include <complex>
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
+#include <map>
+#include <tuple>
 #include <vector>
 
 namespace
@@ -372,6 +379,169 @@ void summed_apply_weighted(bool par, size_t n, size_t n_strings, size_t n_ops, u
     }
 }
 
+// ---------------------------------------------------------------------------
+// String algebra
+// ---------------------------------------------------------------------------
+
+// __pauli.hpp:110-178: the product of two letters (0 = I, 1 = X, 2 = Y, 3 = Z) and
+// its phase, as the multiplication table of the Pauli matrices (row: left letter).
+struct LetterProduct
+{
+    double re, im;
+    uint8_t code;
+};
+constexpr LetterProduct kLetterTable[4][4] = {
+    {{1, 0, 0}, {1, 0, 1}, {1, 0, 2}, {1, 0, 3}},
+    {{1, 0, 1}, {1, 0, 0}, {0, 1, 3}, {0, -1, 2}},
+    {{1, 0, 2}, {0, -1, 3}, {1, 0, 0}, {0, 1, 1}},
+    {{1, 0, 3}, {0, 1, 2}, {0, -1, 1}, {1, 0, 0}},
+};
+
+// __pauli_string.hpp:228-247: letter by letter, the phase multiplied up in complex<double>.
+cplx<double> string_mul(size_t n, uint8_t const *a, uint8_t const *b, uint8_t *out)
+{
+    cplx<double> phase = 1;
+    for (size_t q = 0; q < n; ++q)
+    {
+        LetterProduct const &lp = kLetterTable[a[q]][b[q]];
+        phase *= cplx<double>(lp.re, lp.im);
+        out[q] = lp.code;
+    }
+    return phase;
+}
+
+// __pauli_helpers.hpp:33-58: the words of `weight` non-identity letters, each new
+// letter appended (the first letter most significant), X < Y < Z.
+std::vector<std::vector<uint8_t>> nontrivial_words(size_t weight)
+{
+    if (weight == 0)
+        return {};
+    std::vector<std::vector<uint8_t>> words{{1}, {2}, {3}};
+    for (size_t i = 1; i < weight; ++i)
+    {
+        std::vector<std::vector<uint8_t>> next;
+        for (auto const &w : words)
+            for (uint8_t l = 1; l <= 3; ++l)
+            {
+                auto e = w;
+                e.push_back(l);
+                next.push_back(std::move(e));
+            }
+        words = std::move(next);
+    }
+    return words;
+}
+
+// __pauli_helpers.hpp:68-89: k of n positions, from the bitmask with k leading ones
+// through every previous permutation of it.
+std::vector<std::vector<size_t>> idx_combinations(size_t n, size_t k)
+{
+    std::vector<std::vector<size_t>> out;
+    std::vector<int> mask(n, 0);
+    for (size_t i = 0; i < k && i < n; ++i)
+        mask[i] = 1;
+    do
+    {
+        std::vector<size_t> c;
+        for (size_t i = 0; i < n; ++i)
+            if (mask[i])
+                c.push_back(i);
+        out.push_back(std::move(c));
+    } while (std::prev_permutation(mask.begin(), mask.end()));
+    return out;
+}
+
+// __pauli_helpers.hpp:99-153: every string of weight 0..max_weight; within a weight, words
+// outer and positions inner.
+std::vector<std::vector<uint8_t>> strings_max_weight(size_t n, size_t max_weight)
+{
+    std::vector<std::vector<uint8_t>> out;
+    for (size_t w = 0; w <= max_weight; ++w)
+    {
+        if (w == 0)
+        {
+            out.emplace_back(n, uint8_t(0));
+            continue;
+        }
+        auto const words = nontrivial_words(w);
+        auto const combos = idx_combinations(n, w);
+        for (auto const &word : words)
+            for (auto const &c : combos)
+            {
+                std::vector<uint8_t> str(n, 0);
+                for (size_t t = 0; t < c.size(); ++t)
+                    str[c[t]] = word[t];
+                out.push_back(std::move(str));
+            }
+    }
+    return out;
+}
+
+// __summed_pauli_op.hpp:197-268.  coeffs is (N x K); returns the squared strings and
+// their (A x K) coefficients.  Part 2 builds T_aij serially in (i, j) order; part 4
+// sums each (a, k) over T_a in that order (OpenMP over (a, k), as the reference).
+template <class T>
+void summed_square(size_t n, size_t N, size_t K, uint8_t const *codes, cplx<T> const *coeffs,
+                   std::vector<std::vector<uint8_t>> &sq, std::vector<cplx<T>> &coeffs_sq)
+{
+    size_t weight = 0;
+    for (size_t i = 0; i < N; ++i)
+    {
+        size_t w = 0;
+        for (size_t q = 0; q < n; ++q)
+            w += codes[i * n + q] != 0;
+        weight = std::max(weight, w);
+    }
+    sq = strings_max_weight(n, std::min(n, 2 * weight));
+    std::map<std::vector<uint8_t>, size_t> index;
+    for (size_t a = 0; a < sq.size(); ++a)
+        index[sq[a]] = a;
+    std::vector<std::vector<std::tuple<size_t, size_t, cplx<T>>>> t_aij(sq.size());
+    std::vector<uint8_t> prod(n);
+    for (size_t i = 0; i < N; ++i)
+        for (size_t j = 0; j < N; ++j)
+        {
+            cplx<T> const phase(string_mul(n, codes + i * n, codes + j * n, prod.data()));
+            t_aij[index.at(prod)].emplace_back(i, j, phase);
+        }
+    std::vector<cplx<T>> ct(K * N);
+    for (size_t i = 0; i < N; ++i)
+        for (size_t k = 0; k < K; ++k)
+            ct[k * N + i] = coeffs[i * K + k];
+    coeffs_sq.assign(sq.size() * K, cplx<T>(0));
+#pragma omp parallel for schedule(dynamic) collapse(2)
+    for (size_t a = 0; a < sq.size(); ++a)
+        for (size_t k = 0; k < K; ++k)
+            for (auto const &[i, j, phase] : t_aij[a])
+                coeffs_sq[a * K + k] += phase * ct[k * N + i] * ct[k * N + j];
+}
+
+// __pauli_op.hpp:237-281 on one thread: products in (i, j) order, equal strings merged
+// (phase * c1_i * c2_j added in that order).  The reference lists the merged strings in
+// the iteration order of per-thread unordered_maps, which is unspecified; here they come
+// in order of first occurrence.
+void op_mul(size_t n, size_t N1, uint8_t const *codes1, cplx<double> const *c1, size_t N2, uint8_t const *codes2,
+            cplx<double> const *c2, std::vector<std::vector<uint8_t>> &strs, std::vector<cplx<double>> &coeffs)
+{
+    std::map<std::vector<uint8_t>, size_t> index;
+    std::vector<uint8_t> prod(n);
+    for (size_t i = 0; i < N1; ++i)
+        for (size_t j = 0; j < N2; ++j)
+        {
+            cplx<double> const phase = string_mul(n, codes1 + i * n, codes2 + j * n, prod.data());
+            cplx<double> const c = phase * c1[i] * c2[j];
+            auto const it = index.find(prod);
+            if (it == index.end())
+            {
+                index.emplace(prod, strs.size());
+                strs.push_back(prod);
+                coeffs.push_back(c);
+            }
+            else
+                coeffs[it->second] += c;
+        }
+}
+
 template <class T> cplx<T> *C(void *p)
 {
     return static_cast<cplx<T> *>(p);
@@ -505,6 +675,70 @@ void fpo_summed_apply_weighted(int dtype, int par, size_t n, size_t n_strings, s
                                                static_cast<double const *>(data), C<double>(out), n_states),
                  summed_apply_weighted<float>(par, n, n_strings, n_ops, codes, CC<float>(coeffs), CC<float>(in),
                                               static_cast<float const *>(data), C<float>(out), n_states));
+}
+
+// Writes the strings of calculate_pauli_strings_max_weight(n, max_weight) (codes, n per
+// string) when `codes_out` is not NULL; returns their count.
+size_t fpo_strings_max_weight(size_t n, size_t max_weight, uint8_t *codes_out)
+{
+    auto const v = strings_max_weight(n, max_weight);
+    if (codes_out)
+        for (size_t a = 0; a < v.size(); ++a)
+            std::memcpy(codes_out + a * n, v[a].data(), n);
+    return v.size();
+}
+
+// Product of two strings: writes the product's codes and its phase (re, im).
+void fpo_string_mul(size_t n, uint8_t const *a, uint8_t const *b, uint8_t *codes_out, double *phase_out)
+{
+    cplx<double> const p = string_mul(n, a, b, codes_out);
+    phase_out[0] = p.real();
+    phase_out[1] = p.imag();
+}
+
+// SummedPauliOp::square: returns the count A of squared strings; when the outputs are
+// not NULL, writes their codes (A x n) and coefficients (A x K complex of `dtype`).
+size_t fpo_summed_square(int dtype, size_t n, size_t N, size_t K, uint8_t const *codes, void const *coeffs_nk,
+                         uint8_t *codes_out, void *coeffs_out)
+{
+    std::vector<std::vector<uint8_t>> sq;
+    auto emit = [&](auto const &c) {
+        if (codes_out)
+            for (size_t a = 0; a < sq.size(); ++a)
+                std::memcpy(codes_out + a * n, sq[a].data(), n);
+        if (coeffs_out)
+            std::memcpy(coeffs_out, c.data(), c.size() * sizeof(c[0]));
+    };
+    if (dtype == 0)
+    {
+        std::vector<cplx<double>> c;
+        summed_square<double>(n, N, K, codes, CC<double>(coeffs_nk), sq, c);
+        emit(c);
+    }
+    else
+    {
+        std::vector<cplx<float>> c;
+        summed_square<float>(n, N, K, codes, CC<float>(coeffs_nk), sq, c);
+        emit(c);
+    }
+    return sq.size();
+}
+
+// PauliOp * PauliOp (complex128): returns the count of merged strings; when the outputs
+// are not NULL, writes their codes and coefficients (first-occurrence order).
+size_t fpo_op_mul(size_t n, size_t N1, uint8_t const *codes1, double const *c1, size_t N2, uint8_t const *codes2,
+                  double const *c2, uint8_t *codes_out, double *coeffs_out)
+{
+    std::vector<std::vector<uint8_t>> strs;
+    std::vector<cplx<double>> coeffs;
+    op_mul(n, N1, codes1, reinterpret_cast<cplx<double> const *>(c1), N2, codes2,
+           reinterpret_cast<cplx<double> const *>(c2), strs, coeffs);
+    if (codes_out)
+        for (size_t a = 0; a < strs.size(); ++a)
+            std::memcpy(codes_out + a * n, strs[a].data(), n);
+    if (coeffs_out)
+        std::memcpy(coeffs_out, coeffs.data(), coeffs.size() * sizeof(cplx<double>));
+    return strs.size();
 }
 
 } // extern "C"

@@ -64,6 +64,14 @@ def lib() -> ctypes.CDLL:
             L.fpo_summed_expectation.argtypes = [i, i, sz, sz, sz, u8p, vp, vp, vp, sz]
             L.fpo_summed_apply.argtypes = [i, i, sz, sz, sz, u8p, vp, vp, vp, sz]
             L.fpo_summed_apply_weighted.argtypes = [i, i, sz, sz, sz, u8p, vp, vp, vp, vp, sz]
+            L.fpo_strings_max_weight.argtypes = [sz, sz, u8p]
+            L.fpo_strings_max_weight.restype = sz
+            L.fpo_string_mul.argtypes = [sz, u8p, u8p, u8p, vp]
+            L.fpo_string_mul.restype = None
+            L.fpo_summed_square.argtypes = [i, sz, sz, sz, u8p, vp, u8p, vp]
+            L.fpo_summed_square.restype = sz
+            L.fpo_op_mul.argtypes = [sz, sz, u8p, vp, sz, u8p, vp, u8p, vp]
+            L.fpo_op_mul.restype = sz
             for f in (L.fpo_sparse_repr, L.fpo_string_apply, L.fpo_string_apply_batch,
                       L.fpo_string_expectation, L.fpo_op_apply, L.fpo_op_expectation,
                       L.fpo_summed_expectation, L.fpo_summed_apply,
@@ -207,6 +215,56 @@ def summed_apply_weighted(strings, coeffs_nk, states, data, out, par=False, dtyp
                                     coeffs_nk.shape[1], _ptr(codes), _ptr(coeffs_nk),
                                     _ptr(states), _ptr(data), _ptr(out), states.shape[1])
     return out
+
+
+def strings_max_weight(n: int, max_weight: int) -> list[str]:
+    """calculate_pauli_strings_max_weight (__pauli_helpers.hpp:147-153) as strings."""
+    count = lib().fpo_strings_max_weight(n, max_weight, None)
+    codes = np.zeros((count, n), np.uint8)
+    lib().fpo_strings_max_weight(n, max_weight, _ptr(codes))
+    letters = np.array(list("IXYZ"))
+    return ["".join(letters[row]) for row in codes]
+
+
+def string_mul(a: str, b: str) -> tuple[complex, str]:
+    """PauliString * PauliString (__pauli_string.hpp:228-247): (phase, product)."""
+    ca, cb = codes_of(a)[0], codes_of(b)[0]
+    out = np.zeros_like(ca)
+    ph = np.zeros(2, np.float64)
+    lib().fpo_string_mul(ca.size, _ptr(ca), _ptr(cb), _ptr(out), _ptr(ph))
+    return complex(ph[0], ph[1]), "".join("IXYZ"[c] for c in out)
+
+
+def summed_square(strings, coeffs_nk, dtype=C128) -> tuple[list[str], np.ndarray]:
+    """SummedPauliOp::square (__summed_pauli_op.hpp:197-268): the squared strings (in
+    calculate_pauli_strings_max_weight order) and their (A, K) coefficients."""
+    codes = codes_of(strings)
+    N, n = codes.shape
+    dt = _cdtype(dtype)
+    coeffs_nk = np.ascontiguousarray(coeffs_nk, dtype=dt)
+    K = coeffs_nk.shape[1]
+    A = lib().fpo_summed_square(dtype, n, N, K, _ptr(codes), _ptr(coeffs_nk), None, None)
+    out_codes = np.zeros((A, n), np.uint8)
+    out = np.zeros((A, K), dt)
+    lib().fpo_summed_square(dtype, n, N, K, _ptr(codes), _ptr(coeffs_nk), _ptr(out_codes), _ptr(out))
+    letters = np.array(list("IXYZ"))
+    return ["".join(letters[row]) for row in out_codes], out
+
+
+def op_mul(strings1, c1, strings2, c2) -> tuple[list[str], np.ndarray]:
+    """PauliOp * PauliOp (__pauli_op.hpp:237-281) on one thread: merged product strings in
+    order of first occurrence and their complex128 coefficients."""
+    a, b = codes_of(strings1), codes_of(strings2)
+    c1 = np.ascontiguousarray(c1, np.complex128)
+    c2 = np.ascontiguousarray(c2, np.complex128)
+    n = a.shape[1]
+    args = (n, a.shape[0], _ptr(a), _ptr(c1), b.shape[0], _ptr(b), _ptr(c2))
+    count = lib().fpo_op_mul(*args, None, None)
+    out_codes = np.zeros((count, n), np.uint8)
+    out = np.zeros(count, np.complex128)
+    lib().fpo_op_mul(*args, _ptr(out_codes), _ptr(out))
+    letters = np.array(list("IXYZ"))
+    return ["".join(letters[row]) for row in out_codes], out
 
 
 def set_threads(n: int) -> None:

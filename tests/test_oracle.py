@@ -298,3 +298,88 @@ def test_planted_mistakes_are_caught():
     # the literal form i^{nY} (-1)^{popcount(j&z)} (sign of the phase flipped for odd nY)
     kc, mc = closed_form("YII")
     assert not np.allclose(np.conj(mc), mc)
+
+
+# ---------------------------------------------------------------------------
+# String algebra: products, the enumeration order, PauliOp * PauliOp, square
+# ---------------------------------------------------------------------------
+def test_string_mul_letters_match_cayley_table():
+    for (a, b), (phase, c) in CAYLEY.items():
+        assert O.string_mul(a, b) == (phase, c)
+
+
+def test_string_mul_matches_dense_products():
+    strings = ["".join(p) for p in it.product("IXYZ", repeat=3)]
+    for a in strings:
+        for b in strings[::3]:
+            phase, c = O.string_mul(a, b)
+            np.testing.assert_array_equal(phase * O.dense_string(c), O.dense_string(a) @ O.dense_string(b))
+
+
+def test_strings_max_weight_known_answers():
+    # fast_pauli/cpp/tests/test_pauli_helpers.cpp:42-51, 110-166; counts sum_w 3^w C(n, w)
+    assert O.strings_max_weight(2, 1) == ["II", "XI", "IX", "YI", "IY", "ZI", "IZ"]
+    r = O.strings_max_weight(4, 2)
+    w2 = r[1 + 12:]
+    assert len(w2) == 54 and w2[0] == "XXII" and w2[1] == "XIXI" and w2[53] == "IIZZ"
+    assert len(O.strings_max_weight(12, 2)) == 631
+    from math import comb
+    for n in range(1, 7):
+        for w in range(0, n + 1):
+            v = O.strings_max_weight(n, w)
+            assert len(v) == sum(3 ** k * comb(n, k) for k in range(w + 1))
+            assert len(set(v)) == len(v)
+            assert [s.count("I") for s in v] == sorted(s.count("I") for s in v)[::-1]
+
+
+@pytest.mark.parametrize("n,K", [(1, 1), (2, 3), (3, 2), (4, 4)])
+def test_summed_square_matches_dense(n, K):
+    """A_k^2 from the oracle's square equals the dense product (test_summed_pauli_op.py:227-241,
+    test_summed_pauli_op.cpp:411-470)."""
+    strings = O.strings_max_weight(n, 2)
+    c = rand_c(len(strings), K, seed=n * 10 + K)
+    sq, csq = O.summed_square(strings, c)
+    assert sq == O.strings_max_weight(n, min(n, 4))
+    for k in range(K):
+        a = O.dense_op(c[:, k], strings)
+        np.testing.assert_allclose(O.dense_op(csq[:, k], sq), a @ a, atol=1e-12)
+
+
+def test_summed_square_random_strings_and_c64():
+    rng = np.random.default_rng(5)
+    n, N, K = 5, 14, 3
+    strings = ["".join(rng.choice(list("IXYZ"), n)) for _ in range(N)]
+    c = rng.standard_normal((N, K)) + 1j * rng.standard_normal((N, K))
+    sq, csq = O.summed_square(strings, c)
+    for k in range(K):
+        a = O.dense_op(c[:, k], strings)
+        np.testing.assert_allclose(O.dense_op(csq[:, k], sq), a @ a, atol=1e-11)
+    sq32, c32 = O.summed_square(strings, c.astype(np.complex64), dtype=O.C64)
+    assert sq32 == sq and c32.dtype == np.complex64
+    np.testing.assert_allclose(c32, csq, atol=1e-4)
+
+
+def test_op_mul_matches_dense():
+    rng = np.random.default_rng(9)
+    s1 = ["".join(rng.choice(list("IXYZ"), 3)) for _ in range(7)]
+    s2 = ["".join(rng.choice(list("IXYZ"), 3)) for _ in range(5)]
+    c1, c2 = rand_c(7, seed=1), rand_c(5, seed=2)
+    strs, c = O.op_mul(s1, c1, s2, c2)
+    assert len(set(strs)) == len(strs)
+    np.testing.assert_allclose(O.dense_op(c, strs), O.dense_op(c1, s1) @ O.dense_op(c2, s2), atol=1e-13)
+
+
+def test_square_with_planted_phase_mistake_fails_dense():
+    """The dense check is sensitive to the phases: the same T_aij contraction with the
+    phases dropped (a plausible mistake) is caught.  (Conjugated phases would not be: in
+    a square, P_j P_i carries the conjugate phase of P_i P_j with the same h_i h_j.)"""
+    strings = O.strings_max_weight(2, 2)
+    c = rand_c(len(strings), 1, seed=4)[:, 0]
+    a = O.dense_op(c, strings)
+    index = {s: i for i, s in enumerate(O.strings_max_weight(2, 2))}
+    wrong = np.zeros(len(index), np.complex128)
+    for i, si in enumerate(strings):
+        for j, sj in enumerate(strings):
+            phase, p = O.string_mul(si, sj)
+            wrong[index[p]] += c[i] * c[j]
+    assert not np.allclose(O.dense_op(wrong, list(index)), a @ a, atol=1e-6)

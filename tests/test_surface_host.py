@@ -249,33 +249,6 @@ def test_summed_ctor_coeffs_split_and_dense():
             np.testing.assert_array_equal(op.coeffs, new)
 
 
-def test_summed_square():
-    rng = np.random.default_rng(7)
-    strings = fp.helpers.calculate_pauli_strings_max_weight(3, 1)
-    N, K = len(strings), 2
-    c = rng.standard_normal((N, K)) + 0j
-    op = fp.SummedPauliOp(strings, c)
-    sq = op.square()
-    dense, dsq = op.to_tensor(), sq.to_tensor()
-    for k in range(K):
-        np.testing.assert_allclose(dsq[k], dense[k] @ dense[k], atol=1e-12)
-
-
-def test_summed_square_random_strings():
-    """square() (products found by bit masks, phases letter by letter) equals A_k @ A_k
-    for random strings with every letter, and the fallback by string keys agrees."""
-    rng = np.random.default_rng(11)
-    n, N, K = 5, 12, 3
-    letters = np.array(list("IXYZ"))
-    strings = sorted({"".join(rng.choice(letters, n)) for _ in range(N)})
-    c = rng.standard_normal((len(strings), K)) + 1j * rng.standard_normal((len(strings), K))
-    op = fp.SummedPauliOp(strings, c)
-    sq = op.square()
-    dense, dsq = op.to_tensor(), sq.to_tensor()
-    for k in range(K):
-        np.testing.assert_allclose(dsq[k], dense[k] @ dense[k], atol=1e-12)
-
-
 def test_summed_pickle_clone():
     strings = ["XI", "IZ", "YY"]
     c = np.arange(6, dtype=np.complex128).reshape(3, 2)
