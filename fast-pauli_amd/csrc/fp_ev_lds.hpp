@@ -42,6 +42,21 @@ namespace fpl
 constexpr int kChunk = 8;
 constexpr int kThreads = 1024;
 
+// A class of strings with equal x and equal half of w is evaluated by one
+// Walsh-Hadamard transform of G over the leader rows when it holds at least
+// kWhtMin strings (split into classes of at most kWhtMax):
+//     ev_s = scale_s * sum_leaders (-1)^{parity(a & Z_s)} G(a) = scale_s * W[compress(Z_s)],
+// one pass over the rows and (d-1) butterfly levels instead of one pass per string.
+// One wave transforms a class on its own, in parts of kWhtPartBytes of G held in
+// a private LDS slice (no block barrier); the parts are combined per string by
+// their top index bits.
+constexpr uint32_t kWhtMin = 12;
+constexpr uint32_t kWhtMax = 256;        // 4 strings per lane
+constexpr uint32_t kWhtMinRowsLog2 = 10; // columns shorter than this keep the per-string loop
+constexpr uint32_t kWhtPartBytes = 2048;
+constexpr uint32_t kWhtWaves = 15;       // waves with a slice (the last wave takes per-string chunks only)
+constexpr uint32_t kWhtLdsBytes = kWhtWaves * kWhtPartBytes;
+
 enum : uint32_t
 {
     kAllRows = 0,
@@ -81,6 +96,129 @@ struct EvStr
     uint32_t zc;   // coset kernel: z compressed onto the coset-index bits (0 otherwise)
     double scale;  // eps (kAllRows) or 2 eps (kPairs)
 };
+
+// v with bit 0 of `bits` inserted as a zero at position h
+__device__ __forceinline__ uint32_t insert_zero(uint32_t v, uint32_t h)
+{
+    return ((v >> h) << (h + 1)) | (v & ((1u << h) - 1u));
+}
+
+// Orders this wave's LDS accesses: those before complete before those after, for
+// every lane (the wave's LDS instructions execute in order; this keeps the
+// compiler from moving them across).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Walsh-Hadamard evaluation of one class (see kWhtMin) by one wave, on a column
+// of 2^d rows in LDS, with `scr` its kWhtPartBytes slice.  Leaders: every row
+// when x = 0, else the rows with bit h clear; G = |psi|^2, Re w or Im w as in the
+// per-string loop.  Calls emit(s, value) on the lane of string s (s = lane + 64 j)
+// with value = sum_leaders (-1)^{parity(a & Z_s)} G(a) (the caller scales it).
+template <class T, class Emit>
+__device__ void wht_class(typename fpk::VecOf<T, 1>::type const *col, uint32_t d, uint32_t lane, EvChunk const &ch,
+                          EvStr const *__restrict__ strs, typename fpk::RowAcc<T>::type *scr, Emit emit)
+{
+    typedef typename fpk::RowAcc<T>::type A;
+    typedef typename fpk::VecOf<T, 1>::type V;
+    constexpr uint32_t kPart = kWhtPartBytes / sizeof(A); // 256 (FP64) or 512 (FP32) values
+    constexpr uint32_t kPartLog2 = kPart == 512 ? 9u : 8u;
+    constexpr uint32_t kVals = kPart / 64;                // per lane: 4 or 8
+    constexpr uint32_t kRb = kVals == 8 ? 3u : 2u;        // index bits per register round
+    constexpr int kRes = kWhtMax / 64;
+    uint32_t const x = ch.x, h = ch.h;
+    uint32_t const L = x ? d - 1 : d;
+    uint32_t const gb = L < kPartLog2 ? L : kPartLog2;
+    uint32_t const gsize = 1u << gb, gmask = gsize - 1u, pb = L - gb;
+    A res[kRes];
+    uint32_t zc[kRes];
+#pragma unroll
+    for (int j = 0; j < kRes; ++j)
+    {
+        res[j] = A(0);
+        uint32_t const s = lane + 64u * uint32_t(j);
+        uint32_t const z = s < ch.count ? strs[ch.first + s].z : 0u;
+        zc[j] = x ? ((z & ((1u << h) - 1u)) | ((z >> (h + 1)) << h)) : z;
+    }
+    for (uint32_t part = 0; part < (1u << pb); ++part)
+    {
+        // G of this part's leaders into the slice
+#pragma unroll
+        for (uint32_t m = 0; m < kVals; ++m)
+        {
+            uint32_t const i = lane + 64u * m;
+            if (i < gsize)
+            {
+                uint32_t const ap = (part << gb) | i;
+                uint32_t const a = x ? insert_zero(ap, h) : ap;
+                V const v = col[a];
+                A const ar = A(v[0]), ai = A(v[1]);
+                A g;
+                if (x == 0)
+                    g = fma(ar, ar, ai * ai);
+                else
+                {
+                    V const w = col[a ^ x];
+                    A const br = A(w[0]), bi = A(w[1]);
+                    g = ch.imag ? fma(ar, bi, -(ai * br)) : fma(ar, br, ai * bi);
+                }
+                scr[i] = g;
+            }
+        }
+        // W[z] = sum_i (-1)^{parity(i & z)} G[i]: rounds of kRb index bits in registers
+        for (uint32_t b0 = 0; b0 < gb; b0 += kRb)
+        {
+            uint32_t const rb = gb - b0 < kRb ? gb - b0 : kRb;
+            uint32_t const n_groups = gsize >> rb;
+            wave_lds_sync();
+            for (uint32_t grp = lane; grp < n_groups; grp += 64)
+            {
+                // group base: the group index with rb zero bits inserted at b0
+                uint32_t const base = ((grp >> b0) << (b0 + rb)) | (grp & ((1u << b0) - 1u));
+                A v[kVals];
+#pragma unroll
+                for (uint32_t m = 0; m < kVals; ++m)
+                    v[m] = m < (1u << rb) ? scr[base | (m << b0)] : A(0);
+#pragma unroll
+                for (uint32_t lvl = 0; lvl < kRb; ++lvl)
+#pragma unroll
+                    for (uint32_t m = 0; m < kVals; ++m)
+                        if (!((m >> lvl) & 1u))
+                        {
+                            A const u = v[m], w = v[m | (1u << lvl)];
+                            v[m] = u + w;
+                            v[m | (1u << lvl)] = u - w;
+                        }
+#pragma unroll
+                for (uint32_t m = 0; m < kVals; ++m)
+                    if (m < (1u << rb))
+                        scr[base | (m << b0)] = v[m];
+            }
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int j = 0; j < kRes; ++j)
+        {
+            uint32_t const s = lane + 64u * uint32_t(j);
+            if (s < ch.count)
+            {
+                A const t = scr[zc[j] & gmask];
+                res[j] += (__popc((zc[j] >> gb) & part) & 1) ? -t : t;
+            }
+        }
+        wave_lds_sync(); // the next part overwrites the slice
+    }
+#pragma unroll
+    for (int j = 0; j < kRes; ++j)
+    {
+        uint32_t const s = lane + 64u * uint32_t(j);
+        if (s < ch.count)
+            emit(s, res[j]);
+    }
+}
 
 __device__ __forceinline__ float flip_sign(float v, uint32_t bits)
 {
@@ -146,6 +284,7 @@ __device__ __forceinline__ void chunk_rows_n(typename fpk::VecOf<T, 1>::type con
 {
     typedef typename fpk::RowAcc<T>::type A;
     typedef typename fpk::VecOf<T, 1>::type V;
+    constexpr bool SUMFIRST = true;
     uint32_t const x = ch.x;
     bool const pairs = ch.mode != kAllRows;
     uint32_t const hh = ch.mode == kPairs ? ch.h - 6 : 0;
@@ -193,11 +332,10 @@ __device__ __forceinline__ void chunk_rows_n(typename fpk::VecOf<T, 1>::type con
 #pragma unroll
         for (int s = 0; s < CNT; ++s)
             flip[s] = uint32_t(__popc(uq0 & zhi[s]) & 1) << 31;
-        if constexpr (sizeof(A) == 4)
+        if constexpr (SUMFIRST)
         {
-            // FP32: the U terms of a string first, with its per-chunk signs, then this
-            // step's sign once: one scalar sign per string instead of one per term
-            // (the FP32 form is bound by scalar instructions, FP64 by the vector FMAs)
+            // the U terms of a string first, with its per-chunk signs, then this step's
+            // sign once: one scalar sign per string instead of one per term
             A tsum[CNT];
 #pragma unroll
             for (int s = 0; s < CNT; ++s)
@@ -273,11 +411,14 @@ __device__ __forceinline__ void chunk_rows(typename fpk::VecOf<T, 1>::type const
         chunk_rows_k<T, U, 1>(col, lane, ch, n_blocks64, zhi, acc);
 }
 
+// chunks[0, n_wht) are transform classes (wht_class), the rest per-string chunks.
+// The LDS holds the column, then (with n_wht > 0) the transform buffers.
 template <class T>
 __global__ __launch_bounds__(kThreads) void k_ev_lds(typename fpk::VecOf<T, 1>::type const *__restrict__ states_T,
                                                      uint32_t dim, EvChunk const *__restrict__ chunks,
-                                                     uint32_t n_chunks, EvStr const *__restrict__ strs,
-                                                     uint64_t n_states, T *__restrict__ ev)
+                                                     uint32_t n_chunks, uint32_t n_wht,
+                                                     EvStr const *__restrict__ strs, uint64_t n_states,
+                                                     T *__restrict__ ev)
 {
     typedef typename fpk::VecOf<T, 1>::type V;
     typedef typename fpk::RowAcc<T>::type A;
@@ -288,14 +429,41 @@ __global__ __launch_bounds__(kThreads) void k_ev_lds(typename fpk::VecOf<T, 1>::
     V const *src = states_T + t * dim;
     for (uint32_t i = threadIdx.x; i < dim; i += kThreads)
         col[i] = src[i];
-    // chunks [c_begin, c_end) of this block (gridDim.y splits the chunk list)
-    uint32_t const per = (n_chunks + gridDim.y - 1) / gridDim.y;
-    uint32_t const c_begin = blockIdx.y * per;
+    // per-string chunks [c_begin, c_end) of this block (gridDim.y splits the list)
+    uint32_t const n_reg = n_chunks - n_wht;
+    uint32_t const per = (n_reg + gridDim.y - 1) / gridDim.y;
+    uint32_t const c_begin = n_wht + blockIdx.y * per;
     uint32_t const c_end = c_begin + per < n_chunks ? c_begin + per : n_chunks;
+    __shared__ uint32_t next_wht;
     if (threadIdx.x == 0)
+    {
+        next_wht = blockIdx.y;
         next_chunk = c_begin;
+    }
     __syncthreads();
     uint32_t const lane = threadIdx.x & 63u;
+    uint32_t const wave = threadIdx.x >> 6;
+    if (n_wht && wave < kWhtWaves)
+    {
+        // transform classes c = y, y + gy, ... of this block, one per wave at a time;
+        // a wave then moves on to the per-string chunks without waiting for the others
+        A *scr = reinterpret_cast<A *>(smem + size_t(dim) * sizeof(V) + wave * kWhtPartBytes);
+        uint32_t const d = 31u - uint32_t(__builtin_clz(dim));
+        while (true)
+        {
+            uint32_t c = 0;
+            if (lane == 0)
+                c = atomicAdd(&next_wht, gridDim.y);
+            c = __builtin_amdgcn_readfirstlane(__shfl(c, 0, 64));
+            if (c >= n_wht)
+                break;
+            EvChunk const ch = chunks[c];
+            wht_class<T>(col, d, lane, ch, strs, scr, [&](uint32_t s, A v) {
+                EvStr const st = strs[ch.first + s];
+                ev[uint64_t(st.out) * n_states + t] = T(double(v) * st.scale);
+            });
+        }
+    }
     uint32_t const n_blocks64 = dim >> 6;
     while (true)
     {
@@ -372,14 +540,41 @@ __global__ __launch_bounds__(kThreads) void k_ev_coset(typename fpk::VecOf<T, 1>
     uint64_t const *bt = btab + grp.btab_off;
     for (uint32_t i = threadIdx.x; i < rows; i += kThreads)
         col[i] = src[cb ^ bt[i >> k] ^ uint64_t(i & lowmask)];
-    uint32_t const per = (grp.n_chunks + gridDim.y - 1) / gridDim.y;
-    uint32_t const c_begin = grp.chunk_first + blockIdx.y * per;
+    uint32_t const n_reg = grp.n_chunks - grp.n_wht;
+    uint32_t const per = (n_reg + gridDim.y - 1) / gridDim.y;
+    uint32_t const c_begin = grp.chunk_first + grp.n_wht + blockIdx.y * per;
     uint32_t const c_last = grp.chunk_first + grp.n_chunks;
     uint32_t const c_end = c_begin + per < c_last ? c_begin + per : c_last;
+    // the group's first n_wht chunks are transform classes (wht_class), one per wave at a
+    // time; a wave then moves on to the per-string chunks without waiting for the others
+    __shared__ uint32_t next_wht;
     if (threadIdx.x == 0)
+    {
+        next_wht = blockIdx.y;
         next_chunk = c_begin;
+    }
     __syncthreads();
     uint32_t const lane = threadIdx.x & 63u;
+    uint32_t const wave = threadIdx.x >> 6;
+    if (grp.n_wht && wave < kWhtWaves)
+    {
+        A *scr = reinterpret_cast<A *>(smem + size_t(rows) * sizeof(V) + wave * kWhtPartBytes);
+        while (true)
+        {
+            uint32_t c = 0;
+            if (lane == 0)
+                c = atomicAdd(&next_wht, gridDim.y);
+            c = __builtin_amdgcn_readfirstlane(__shfl(c, 0, 64));
+            if (c >= grp.n_wht)
+                break;
+            EvChunk const ch = chunks[grp.chunk_first + c];
+            wht_class<T>(col, d, lane, ch, strs, scr, [&](uint32_t s, A v) {
+                EvStr const st = strs[ch.first + s];
+                double const dv = double(v) * st.scale;
+                partial[(uint64_t(q) * n_strings + st.out) * n_states + t] = (__popc(q & st.zc) & 1) ? -dv : dv;
+            });
+        }
+    }
     uint32_t const n_blocks64 = rows >> 6;
     while (true)
     {

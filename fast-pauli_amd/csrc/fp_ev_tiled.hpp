@@ -71,6 +71,8 @@ struct TileGroup
                         // form (expectation tables)
     uint32_t str_first; // first ShStr of the group (expectation tables)
     uint32_t n_str;     // strings of the group
+    uint32_t n_wht;     // per-string EV matrix tables: the first n_wht chunks are transform classes
+    uint32_t pad;
 };
 
 // Strings of a shared-leader group: every local x mask has bit k set, so the

@@ -280,8 +280,10 @@ void fill_coeff_tables(fp_hip_ops *o, ArenaLayout const &L)
 
 // Groups strings with equal x and equal half of w (the parity of nY) into
 // chunks of at most fpl::kChunk for the LDS expectation kernel, largest work
-// first.  Returns the number of chunks.
-uint32_t build_ev_chunks(fp_hip_ops const *o, fpl::EvChunk *chunks, fpl::EvStr *strs)
+// first.  Classes of at least fpl::kWhtMin strings (columns of 2^kWhtMinRowsLog2
+// rows and more) become transform classes of at most fpl::kWhtMax strings, listed
+// first (*n_wht of them).  Returns the number of chunks, transform classes included.
+uint32_t build_ev_chunks(fp_hip_ops const *o, fpl::EvChunk *chunks, fpl::EvStr *strs, uint32_t *n_wht)
 {
     size_t const N = o->n_strings;
     std::vector<uint32_t> order(N);
@@ -294,6 +296,7 @@ uint32_t build_ev_chunks(fp_hip_ops const *o, fpl::EvChunk *chunks, fpl::EvStr *
         fpl::EvChunk c;
         std::vector<fpl::EvStr> s;
         double cost;
+        bool wht;
     };
     std::vector<Pending> pend;
     size_t i = 0;
@@ -305,14 +308,17 @@ uint32_t build_ev_chunks(fp_hip_ops const *o, fpl::EvChunk *chunks, fpl::EvStr *
         uint64_t const x = o->x[order[i]];
         uint32_t const h = x ? 63u - uint32_t(__builtin_clzll(x)) : 0u;
         uint32_t const mode = fpl::ev_mode(x, h, uint64_t(1) << o->n_qubits);
-        for (size_t k = i; k < j; k += fpl::kChunk)
+        bool const wht = j - i >= fprt::wht_min_strings() && o->n_qubits >= fpl::kWhtMinRowsLog2;
+        size_t const per_chunk = wht ? fpl::kWhtMax : size_t(fpl::kChunk);
+        for (size_t k = i; k < j; k += per_chunk)
         {
             Pending p{};
+            p.wht = wht;
             p.c.x = uint32_t(x);
             p.c.h = h;
             p.c.mode = mode;
             p.c.imag = o->ny[order[i]] & 1u;
-            p.c.count = uint32_t(std::min<size_t>(fpl::kChunk, j - k));
+            p.c.count = uint32_t(std::min<size_t>(per_chunk, j - k));
             for (size_t m = k; m < k + p.c.count; ++m)
             {
                 uint32_t const sidx = order[m];
@@ -326,7 +332,13 @@ uint32_t build_ev_chunks(fp_hip_ops const *o, fpl::EvChunk *chunks, fpl::EvStr *
         }
         i = j;
     }
-    std::stable_sort(pend.begin(), pend.end(), [](Pending const &a, Pending const &b) { return a.cost > b.cost; });
+    // transform classes first, then per-string chunks, each costliest first
+    std::stable_sort(pend.begin(), pend.end(), [](Pending const &a, Pending const &b) {
+        return a.wht != b.wht ? a.wht : a.cost > b.cost;
+    });
+    *n_wht = 0;
+    for (auto const &p : pend)
+        *n_wht += p.wht ? 1u : 0u;
     uint32_t first = 0;
     for (size_t c = 0; c < pend.size(); ++c)
     {
@@ -394,7 +406,7 @@ fp_status ops_assign(fp_hip_ops *o, size_t n_qubits, size_t n_strings, uint8_t c
         o->coeffs[2 * s + 1] = coeffs ? coeffs[2 * s + 1] : 0.0;
     }
     fill_coeff_tables(o, L);
-    o->n_ev_chunks = build_ev_chunks(o, at<fpl::EvChunk>(h, L.ev_chunks), at<fpl::EvStr>(h, L.ev_strs));
+    o->n_ev_chunks = build_ev_chunks(o, at<fpl::EvChunk>(h, L.ev_chunks), at<fpl::EvStr>(h, L.ev_strs), &o->n_ev_wht);
     if (o->arena_bytes < L.total)
     {
         if (o->d_arena)
@@ -700,7 +712,8 @@ namespace
 bool lds_eligible(fp_hip_ops const *ops, int dtype, size_t dim)
 {
     size_t const esz = dtype == FP_C128 ? 16 : 8;
-    return ops->n_qubits >= 6 && dim * esz <= column_lds_bytes() && ops->n_ev_chunks > 0;
+    size_t const extra = ops->n_ev_wht ? fpl::kWhtLdsBytes : 0;
+    return ops->n_qubits >= 6 && dim * esz + extra <= column_lds_bytes() && ops->n_ev_chunks > 0;
 }
 
 // Scratch bytes of stage 1 (run_strings_expect): the transposed states for the
@@ -725,7 +738,7 @@ fp_status launch_ev_lds(fp_hip_ops const *ops, void const *states, void *ev, siz
     hipLaunchKernelGGL(fpl::k_transpose<V>, tg, dim3(256), 0, st, static_cast<V const *>(states),
                        static_cast<V *>(transposed), uint64_t(dim), uint64_t(n_states));
     FP_LAUNCHED();
-    size_t const lds = dim * sizeof(V);
+    size_t const lds = dim * sizeof(V) + (ops->n_ev_wht ? fpl::kWhtLdsBytes : 0);
     static bool attr_set[2] = {false, false};
     int const which = sizeof(T) == 8 ? 1 : 0;
     if (!attr_set[which])
@@ -739,7 +752,7 @@ fp_status launch_ev_lds(fp_hip_ops const *ops, void const *states, void *ev, siz
         gy = uint32_t(std::min<size_t>(ops->n_ev_chunks, (1024 + n_states - 1) / n_states));
     hipLaunchKernelGGL(fpl::k_ev_lds<T>, dim3(uint32_t(n_states), gy), dim3(fpl::kThreads), lds, st,
                        static_cast<V const *>(transposed), uint32_t(dim), ops->d_ev_chunks, ops->n_ev_chunks,
-                       ops->d_ev_strs, uint64_t(n_states), static_cast<T *>(ev));
+                       ops->n_ev_wht, ops->d_ev_strs, uint64_t(n_states), static_cast<T *>(ev));
     FP_LAUNCHED();
     return FP_OK;
 }

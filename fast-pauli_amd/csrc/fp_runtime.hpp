@@ -7,6 +7,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <cstdlib>
 #include <list>
 #include <mutex>
 #include <vector>
@@ -62,6 +63,21 @@ inline size_t column_lds_bytes()
     return lds_bytes() - 256;
 }
 
+// Smallest class of strings (equal x, equal half of w) that the per-string EV
+// matrix kernels evaluate through a Walsh-Hadamard transform, or
+// FAST_PAULI_WHT_MIN (0 disables the transform).  The transform trades vector
+// and scalar instructions for LDS traffic: the complex128 coset kernel, whose
+// loop is LDS-bound, gains from the largest classes only (measured at n = 14,
+// 2000 weight-3 strings: 2.39 ms with 12, 2.32 ms with 40; complex64 1.26 / 1.35 ms).
+inline uint32_t wht_min_strings(uint32_t dflt = fpl::kWhtMin)
+{
+    char const *v = std::getenv("FAST_PAULI_WHT_MIN");
+    if (!v)
+        return dflt;
+    long const n = std::strtol(v, nullptr, 10);
+    return n <= 0 ? 0xffffffffu : uint32_t(n);
+}
+
 // A launch stays below 2^32 work-items: at most 2^23 blocks of 256 threads.
 constexpr uint64_t kMaxLaunchBlocks = uint64_t(1) << 23;
 
@@ -101,6 +117,7 @@ struct TileTables
 {
     int kind = 0;
     uint32_t k = 0, dh = 0, n_groups = 0, max_chunks = 0;
+    uint32_t wht_min = 0; // per-string EV matrix tables: smallest transform class
     bool valid = false;     // built for the current strings and coefficients
     bool on_device = false; // the device copy matches the host image
     std::vector<unsigned char> h;
@@ -117,7 +134,9 @@ struct TileTables
 
 // `expect`: tiles for the expectation kernel (complex128 takes 128-byte row slices), else apply
 TileGeom tile_geom(int dtype, size_t n_qubits, size_t n_states, bool expect);
-fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, TileTables **out);
+// `wht_min`: for kTilesEvMatrix, the smallest class evaluated through a transform (part of the key)
+fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, TileTables **out,
+                      uint32_t wht_min = 0);
 fp_status tile_device(fp_hip_ops const *ops, TileTables *t, hipStream_t stream);
 void invalidate_tiles(fp_hip_ops *ops);
 // Per-string EV matrix over cosets: for columns that do not fit in LDS whole.
@@ -198,6 +217,7 @@ struct fp_hip_ops
     fpl::EvChunk *d_ev_chunks = nullptr; // strings grouped by x for the LDS expectation kernel
     fpl::EvStr *d_ev_strs = nullptr;
     uint32_t n_ev_chunks = 0;
+    uint32_t n_ev_wht = 0; // the first n_ev_wht chunks are transform classes
     double *d_rowsum128 = nullptr; // SummedPauliOp::apply folded coefficients (complex128)
     float *d_rowsum64 = nullptr;
     // contraction operands: W^T as (n_operators x n_strings) real and imaginary parts

@@ -374,11 +374,11 @@ fpt::TileGroup group_header(HostGroup const &hg, std::vector<uint64_t> &btab)
 } // namespace
 
 // Builds (or returns the cached) tables of `ops` of one kind for geometry (k, dh).
-fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, TileTables **out)
+fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, TileTables **out, uint32_t wht_min)
 {
     std::lock_guard<std::mutex> lock(ops->tiles_mu);
     for (auto &t : ops->tiles)
-        if (t.valid && t.kind == kind && t.k == k && t.dh == dh)
+        if (t.valid && t.kind == kind && t.k == k && t.dh == dh && t.wht_min == wht_min)
         {
             *out = &t;
             return FP_OK;
@@ -471,6 +471,7 @@ fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, 
                 fpl::EvChunk c;
                 std::vector<fpl::EvStr> s;
                 double cost;
+                bool wht;
             };
             std::vector<Pending> pend;
             std::vector<LocalStr> mem = hg.mem;
@@ -479,10 +480,17 @@ fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, 
                              [&](LocalStr const &a, LocalStr const &b) { return key(a) < key(b); });
             for (size_t i = 0; i < mem.size();)
             {
+                // classes of at least kWhtMin strings become transform classes
+                size_t cls = i;
+                while (cls < mem.size() && key(mem[cls]) == key(mem[i]))
+                    ++cls;
+                bool const wht = cls - i >= wht_min && k + dh >= fpl::kWhtMinRowsLog2;
+                size_t const per_chunk = wht ? fpl::kWhtMax : size_t(fpl::kChunk);
                 size_t j = i;
-                while (j < mem.size() && j - i < size_t(fpl::kChunk) && key(mem[j]) == key(mem[i]))
+                while (j < mem.size() && j - i < per_chunk && key(mem[j]) == key(mem[i]))
                     ++j;
                 Pending p{};
+                p.wht = wht;
                 p.c.x = mem[i].A;
                 p.c.h = p.c.x ? 31u - uint32_t(__builtin_clz(p.c.x)) : 0u;
                 p.c.mode = fpl::ev_mode(p.c.x, p.c.h, uint64_t(1) << (k + dh));
@@ -500,10 +508,13 @@ fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, 
                 pend.push_back(std::move(p));
                 i = j;
             }
-            std::stable_sort(pend.begin(), pend.end(),
-                             [](Pending const &a, Pending const &b) { return a.cost > b.cost; });
+            // transform classes first, then per-string chunks, each costliest first
+            std::stable_sort(pend.begin(), pend.end(), [](Pending const &a, Pending const &b) {
+                return a.wht != b.wht ? a.wht : a.cost > b.cost;
+            });
             for (auto &p : pend)
             {
+                g.n_wht += p.wht ? 1u : 0u;
                 p.c.first = uint32_t(strs.size() / sizeof(fpl::EvStr));
                 for (auto const &e : p.s)
                     strs.insert(strs.end(), reinterpret_cast<unsigned char const *>(&e),
@@ -549,6 +560,7 @@ fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, 
     t->kind = kind;
     t->k = k;
     t->dh = dh;
+    t->wht_min = wht_min;
     t->n_groups = uint32_t(groups.size());
     t->max_chunks = max_chunks;
     auto up = [](size_t v) { return (v + 255) / 256 * 256; };
@@ -957,6 +969,9 @@ fp_status run_op_apply_tiled(fp_hip_ops const *ops, int dtype, void const *state
 namespace
 {
 
+// Smallest transform class of the complex128 coset kernel (see wht_min_strings).
+constexpr uint32_t kWhtMinCosetC128 = 48;
+
 struct CosetGeom
 {
     uint32_t d, k, n_cos;
@@ -1021,7 +1036,13 @@ fp_status launch_ev_coset(fp_hip_ops const *ops, TileTables *tt, CosetGeom const
     uint32_t gy = 1;
     if (n_states < 1024)
         gy = uint32_t(std::max<size_t>(1, std::min<size_t>(tt->max_chunks, (1024 + n_states - 1) / n_states)));
-    size_t const lds = (size_t(1) << g.d) * sizeof(V);
+    bool any_wht = false;
+    auto const *hgroups = reinterpret_cast<fpt::TileGroup const *>(tt->h.data());
+    for (uint32_t i = 0; i < tt->n_groups; ++i)
+        any_wht = any_wht || hgroups[i].n_wht > 0;
+    size_t const lds = (size_t(1) << g.d) * sizeof(V) + (any_wht ? fpl::kWhtLdsBytes : 0);
+    if (lds > column_lds_bytes())
+        return fail(FP_ERR_RUNTIME, "coset column and transform buffers (%zu bytes) exceed the LDS", lds);
     dim3 const grid(uint32_t(n_states), gy, tt->n_groups * g.n_cos);
     hipLaunchKernelGGL(fpl::k_ev_coset<T>, grid, dim3(fpl::kThreads), lds, st, transposed, uint64_t(dim), g.d, g.k,
                        g.n_cos, tt->groups, tt->btab, static_cast<fpl::EvChunk const *>(tt->chunks),
@@ -1040,7 +1061,8 @@ fp_status run_ev_coset(fp_hip_ops const *ops, int dtype, void const *states, voi
 {
     CosetGeom const g = coset_geom(dtype, ops->n_qubits);
     TileTables *tt = nullptr;
-    fp_status s = tile_tables(ops, kTilesEvMatrix, g.k, g.d - g.k, &tt);
+    fp_status s = tile_tables(ops, kTilesEvMatrix, g.k, g.d - g.k, &tt,
+                              wht_min_strings(dtype == FP_C128 ? kWhtMinCosetC128 : fpl::kWhtMin));
     if (s != FP_OK)
         return s;
     if ((s = tile_device(ops, tt, st)) != FP_OK)

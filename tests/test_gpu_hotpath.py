@@ -556,3 +556,30 @@ def test_strings_expectation_matrix_all_row_modes(hip, monkeypatch, dtype, n, S,
     assert H.rel_err(got, exp) <= tol
     # each row against its own scale: |<P>| <= 1 for normalised states
     assert np.abs(got - exp).max() <= (1e-13 if dtype == C128 else 2e-6)
+
+
+@pytest.mark.parametrize("dtype,n,S", [(C128, 10, 7), (C64, 12, 5), (C128, 13, 33), (C64, 14, 9),
+                                       (C128, 14, 3), (C128, 15, 2), (C64, 16, 4), (C128, 11, 1)])
+def test_strings_expectation_matrix_transform_classes(hip, dtype, n, S):
+    """Classes of >= 12 strings with one x (evaluated through a Walsh-Hadamard transform of the
+    pair values): x = 0, x on the lowest bits, around bit 6 and on the top bit, both halves of w
+    (X or Y letters on x), and a few small classes beside them, against the oracle."""
+    H.say(f"ev matrix transform classes n={n} S={S} dtype={dtype}")
+    rng = np.random.default_rng(1000 + n * 10 + S)
+    strings = []
+    x_sets = [[], [n - 1], [n - 2, n - 1], [n - 7], [n - 8, n - 3], [0], [0, n // 2, n - 1]]
+    for pos in x_sets:
+        for _ in range(30):
+            s = list(rng.choice(list("IZ"), n))
+            for p in pos:
+                s[p] = rng.choice(list("XY"))
+            strings.append("".join(s))
+    strings += ["".join(rng.choice(list("IXYZ"), n)) for _ in range(25)]
+    codes = O.codes_of(strings)
+    psi = syn.random_states(1 << n, S, seed=n + S).astype(H.CDT[dtype])
+    got = H.ev_matrix(_abi.Ops(codes), psi, dtype)
+    p128 = psi.astype(np.complex128)
+    exp = np.stack([O.string_expectation(s, 1, p128, np.zeros(S, np.complex128)).real for s in strings])
+    tol = 1e-12 if dtype == C128 else 1e-5
+    assert H.rel_err(got, exp) <= tol
+    assert np.abs(got - exp).max() <= (1e-13 if dtype == C128 else 2e-6)
