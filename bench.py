@@ -598,10 +598,16 @@ def bench_uneven_batch(args) -> dict:
     # per-op ideal under the streaming model: every op's (strings + 1) state passes at the
     # HBM peak (small states are L2 / Infinity-Cache resident, so the batch can beat it)
     ideal_ms = model_bytes / (HBM_PEAK_GBS * 1e9) * 1e3
+    info = b.info()
+    b.check()
+    # the chosen plan's bytes: coset entries move about 3 state passes per string group
+    plan_bytes = info["plan_bytes"]
     return {"ms": ms, "strings_x_states_per_s": strst / (ms / 1e3), "one_launch_per_op_ms": ms_seq,
             "sum_of_per_op_ideal_ms": ideal_ms, "ideal_over_batch": ideal_ms / ms,
             "model_bytes_per_launch": model_bytes, "model_gbs": model_bytes / (ms / 1e3) / 1e9,
-            "schedule": tune["schedule"], "schedule_cached": tune["cached"], "info": b.info(),
+            "plan_bytes_per_launch": plan_bytes, "plan_gbs": plan_bytes / (ms / 1e3) / 1e9,
+            "plan_frac_hbm": plan_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+            "schedule": tune["schedule"], "schedule_cached": tune["cached"], "info": info,
             "config": "256 PauliOp.apply, n ~ U{8..18}, strings log-uniform 16..512, states ~ U{1..64}, complex64, "
                       "one persistent hipGraph launch"}
 

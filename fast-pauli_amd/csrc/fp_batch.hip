@@ -3,13 +3,16 @@
 //
 // A batch lists (operator set, apply | expectation, states, output) entries of
 // any qubit count, string count, state count and dtype.  The host cuts each
-// entry into tiles, orders the tiles (largest first by default), uploads the
+// entry into work items, orders them (largest first by default), uploads the
 // descriptors, and captures [reset counters, persistent kernel] into a
-// hipGraph that is replayed on every launch.
+// hipGraph that is replayed on every launch.  Large apply entries whose string
+// groups share coset tiles become coset items (one string group, a run of 64 KiB
+// tiles each): about three state passes per group instead of one per string.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -35,19 +38,28 @@ struct BatchEntry
 struct fp_hip_batch
 {
     std::vector<BatchEntry> entries;
-    fp_hip_batch_schedule sched{2048, 1u << 18, 4, FP_BATCH_ORDER_LARGEST_FIRST};
+    // coset items by default for states of 256 MiB and more: below that the states
+    // stay in the Infinity Cache, whose repeated-read rate the streaming items reach
+    fp_hip_batch_schedule sched{2048, 1u << 18, 4, FP_BATCH_ORDER_LARGEST_FIRST, 8, 256u << 10};
     bool use_graph = true;
     // device tables, sized by capacity and reused across rebuilds
     bool built = false;
     fpb::BatchOp *d_ops = nullptr;
     fpb::WorkItem *d_items = nullptr;
-    uint32_t *d_counter = nullptr; // [0] work counter, [1..] per-operator completion counters
+    // [0] work counter, [1, 1 + n_done) per-operator and per-string-group completion
+    // counters (zeroed by every launch), [1 + n_done] error word (zeroed by build)
+    uint32_t *d_counter = nullptr;
     fpk::Dbl2 *d_partials = nullptr;
     size_t cap_ops = 0, cap_items = 0, cap_partials = 0, cap_counter = 0;
     std::vector<fpb::BatchOp> h_ops;
     std::vector<fpb::WorkItem> h_items;
     uint32_t n_items = 0;
     uint32_t n_blocks = 0;
+    uint32_t n_done = 0;     // completion counters reset by every launch
+    bool coset = false;      // some entry runs as coset items: the 512-thread kernel
+    size_t lds = 0;          // dynamic LDS per block
+    size_t coset_entries = 0, coset_groups = 0, coset_items = 0;
+    double plan_bytes = 0.0; // state passes of the plan: coset groups, or one per string
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t capture_stream = nullptr;
@@ -137,11 +149,61 @@ int device_cus()
 
 void enqueue_kernels(fp_hip_batch *b, hipStream_t stream)
 {
-    uint32_t const n_ops = uint32_t(b->h_ops.size());
-    hipLaunchKernelGGL(fpb::k_batch_reset, dim3((n_ops + 1 + fpk::kBlock - 1) / fpk::kBlock), dim3(fpk::kBlock), 0,
-                       stream, b->d_counter, b->d_counter + 1, n_ops);
-    hipLaunchKernelGGL(fpb::k_batch_persistent, dim3(b->n_blocks), dim3(fpk::kBlock), 0, stream, b->d_ops, b->d_items,
-                       b->n_items, b->d_counter, b->d_counter + 1, b->d_partials);
+    uint32_t const n = b->n_done;
+    hipLaunchKernelGGL(fpb::k_batch_reset, dim3((n + 1 + fpk::kBlock - 1) / fpk::kBlock), dim3(fpk::kBlock), 0,
+                       stream, b->d_counter, b->d_counter + 1, n);
+    if (b->coset)
+        hipLaunchKernelGGL(fpb::k_batch_persistent<fpb::kTiledBlock>, dim3(b->n_blocks), dim3(fpb::kTiledBlock),
+                           b->lds, stream, b->d_ops, b->d_items, b->n_items, b->d_counter, b->d_counter + 1,
+                           b->d_partials, b->d_counter + 1 + n);
+    else
+        hipLaunchKernelGGL(fpb::k_batch_persistent<fpk::kBlock>, dim3(b->n_blocks), dim3(fpk::kBlock), b->lds,
+                           stream, b->d_ops, b->d_items, b->n_items, b->d_counter, b->d_counter + 1, b->d_partials,
+                           b->d_counter + 1 + n);
+}
+
+// Persistent blocks per CU that can be resident for the kernel the batch launches.
+int resident_blocks(bool coset, size_t lds)
+{
+    int nb = 0;
+    hipError_t const e =
+        coset ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fpb::k_batch_persistent<fpb::kTiledBlock>,
+                                                             fpb::kTiledBlock, lds)
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fpb::k_batch_persistent<fpk::kBlock>, fpk::kBlock,
+                                                             lds);
+    return (e == hipSuccess && nb > 0) ? nb : 1;
+}
+
+// Coset geometry and string-group tables of an apply entry, when its state is at
+// least `min_kib` KiB and its groups move fewer bytes than the streaming form:
+// (3 G - 1) state passes (2 G with no output read for the first group, plus a
+// read of the output per later group) against N + 1.  `tt` null: streaming.
+fp_status coset_plan(BatchEntry const &e, uint32_t min_kib, fprt::TileGeom *geo, fprt::TileTables **tt)
+{
+    *tt = nullptr;
+    if (e.kind != FP_BATCH_APPLY || min_kib == 0 || e.ops->n_strings < 8)
+        return FP_OK;
+    size_t const esz = e.dtype == FP_C128 ? 16 : 8;
+    if (e.dim * e.n_states * esz < size_t(min_kib) * 1024 || (uint64_t(e.dim) * e.n_states) >> 32 != 0)
+        return FP_OK;
+    fprt::TileGeom const g = fprt::tile_geom(e.dtype, e.ops->n_qubits, e.n_states, false);
+    // the kernel's slot decomposition needs the low k row bits inside one thread's
+    // 2^(9 - lsb) consecutive local rows (rm >= k)
+    uint32_t const log2_block = uint32_t(__builtin_ctz(uint32_t(fpb::kTiledBlock)));
+    if (g.dh == 0 || g.d >= e.ops->n_qubits || g.lsb > log2_block || log2_block - g.lsb < g.k)
+        return FP_OK;
+    fprt::TileTables *t = nullptr;
+    fp_status s = fprt::tile_tables(e.ops, fprt::kTilesOpApply, g.k, g.dh, &t);
+    if (s != FP_OK)
+        return s;
+    double const passes = 3.0 * t->n_groups - (e.accumulate ? 0.0 : 1.0);
+    if (passes > 0.8 * double(e.ops->n_strings + 1))
+        return FP_OK;
+    if ((s = fprt::check_tile_fit(t, g, e.dtype, fpa::kTileBytes)) != FP_OK)
+        return s;
+    *geo = g;
+    *tt = t;
+    return FP_OK;
 }
 
 // Cuts the entries into tiles, uploads the tables on `stream` and (in graph
@@ -152,14 +214,25 @@ fp_status build(fp_hip_batch *b, hipStream_t stream)
     drop_graph(b);
     if (b->entries.empty())
         return fail(FP_ERR_INVALID_ARGUMENT, "batch is empty");
+    b->coset_entries = b->coset_groups = b->coset_items = 0;
+    b->plan_bytes = 0.0;
     std::vector<fpb::BatchOp> ops;
     struct Item
     {
         fpb::WorkItem w;
-        double cost;
+        double cost;  // ordering key of the streaming items
+        double bytes; // estimated memory traffic (interleaving of the two lists)
     };
     std::vector<Item> items;
+    struct CosetOp
+    {
+        uint32_t op, groups;
+        double bytes;
+    };
+    std::vector<CosetOp> coset_ops;
     uint64_t partial_elems = 0;
+    uint32_t n_deps = 0;
+    bool any_expect = false;
     fp_hip_batch_schedule const &sc = b->sched;
     for (size_t i = 0; i < b->entries.size(); ++i)
     {
@@ -191,16 +264,59 @@ fp_status build(fp_hip_batch *b, hipStream_t stream)
         op.dtype = uint32_t(e.dtype);
         op.kind = e.kind == FP_BATCH_APPLY ? fpb::kApply : fpb::kExpect;
         op.accumulate = e.accumulate ? 1u : 0u;
+        fprt::TileGeom geo{};
+        fprt::TileTables *tt = nullptr;
+        fp_status const cs = coset_plan(e, sc.coset_min_kib, &geo, &tt);
+        if (cs != FP_OK)
+            return cs;
+        op.sshift = 64u;
+        uint32_t tiles = 0;
+        if (tt)
+        {
+            fp_status const us = fprt::tile_device(o, tt, stream);
+            if (us != FP_OK)
+                return us;
+            op.coset = 1;
+            op.groups = tt->groups;
+            op.btab = tt->btab;
+            op.astrs = static_cast<fpa::ApplyStr const *>(tt->strs);
+            op.lsb = geo.lsb;
+            op.k = geo.k;
+            op.d = geo.d;
+            op.n_cosets = uint64_t(e.dim) >> geo.d;
+            if ((e.n_states & (e.n_states - 1)) == 0)
+                op.sshift = uint32_t(__builtin_ctzll(e.n_states));
+            uint32_t const cpi = std::max<uint32_t>(sc.coset_tiles, 1);
+            uint64_t const chunks = (e.n_states + (size_t(1) << geo.lsb) - 1) >> geo.lsb;
+            uint64_t const per_group = chunks * ((op.n_cosets + cpi - 1) / cpi);
+            if (per_group > 0xffffffffu || uint64_t(n_deps) + tt->n_groups > 0xffffffu)
+                return fail(FP_ERR_INVALID_ARGUMENT, "batch entry %zu has too many coset items", i);
+            op.items_per_group = uint32_t(per_group);
+            op.dep_base = 0; // set below, after the per-operator counters
+            op.n_tiles = tt->n_groups;
+            n_deps += tt->n_groups;
+            // first group: state read + output write (+ output read when accumulating);
+            // later groups: state read + output read and write
+            b->plan_bytes += double(entry_in_bytes(e)) * (3.0 * tt->n_groups - (e.accumulate ? 0.0 : 1.0));
+            coset_ops.push_back(CosetOp{uint32_t(i), tt->n_groups, double(entry_in_bytes(e))});
+            b->coset_groups += tt->n_groups;
+            ++b->coset_entries;
+            ops.push_back(op);
+            continue;
+        }
         op.ct = uint32_t(std::min<uint64_t>(g.vpr, fpk::kBlock));
         op.rt = fpk::kBlock / op.ct;
-        uint32_t tiles = 0;
+        b->plan_bytes += double(entry_in_bytes(e)) * (e.kind == FP_BATCH_APPLY
+                                                          ? double(o->n_strings) + 1.0 + (e.accumulate ? 1.0 : 0.0)
+                                                          : double(o->n_strings));
         if (e.kind == FP_BATCH_APPLY)
         {
             uint64_t const tv = std::max<uint64_t>(sc.tile_vectors, 64);
             for (uint64_t v0 = 0; v0 < g.n_vec; v0 += tv, ++tiles)
             {
                 uint64_t const cnt = std::min<uint64_t>(tv, g.n_vec - v0);
-                items.push_back({fpb::WorkItem{uint32_t(i), tiles, v0, cnt}, double(cnt) * op.n_strings});
+                items.push_back({fpb::WorkItem{uint32_t(i), tiles, v0, cnt, 0, 0}, double(cnt) * op.n_strings,
+                                 double(cnt) * g.nc * (e.dtype == FP_C128 ? 16 : 8) * (op.n_strings + 1.0)});
             }
         }
         else
@@ -212,10 +328,13 @@ fp_status build(fp_hip_batch *b, hipStream_t stream)
             pairs = (pairs + quantum - 1) / quantum * quantum;
             pairs = std::min<uint64_t>(pairs, op.n_pairs);
             op.partial_base = partial_elems;
+            any_expect = true;
             for (uint64_t p0 = 0; p0 < op.n_pairs; p0 += pairs, ++tiles)
             {
                 uint64_t const cnt = std::min<uint64_t>(pairs, op.n_pairs - p0);
-                items.push_back({fpb::WorkItem{uint32_t(i), tiles, p0, cnt}, 2.0 * double(cnt) * double(per_pair)});
+                double const work = 2.0 * double(cnt) * double(per_pair);
+                items.push_back({fpb::WorkItem{uint32_t(i), tiles, p0, cnt, 0, 0}, work,
+                                 work * g.nc * (e.dtype == FP_C128 ? 16 : 8)});
             }
             partial_elems += uint64_t(tiles) * e.n_states;
         }
@@ -223,27 +342,116 @@ fp_status build(fp_hip_batch *b, hipStream_t stream)
         ops.push_back(op);
     }
     if (sc.order == FP_BATCH_ORDER_LARGEST_FIRST)
-        std::stable_sort(items.begin(), items.end(), [](Item const &a, Item const &c) { return a.cost > c.cost; });
-    std::vector<fpb::WorkItem> work(items.size());
-    double total = 0.0;
-    for (size_t i = 0; i < items.size(); ++i)
     {
-        work[i] = items[i].w;
-        total += items[i].cost;
+        std::stable_sort(items.begin(), items.end(), [](Item const &a, Item const &c) { return a.cost > c.cost; });
+        std::stable_sort(coset_ops.begin(), coset_ops.end(),
+                         [](CosetOp const &a, CosetOp const &c) { return a.bytes > c.bytes; });
     }
-    b->est_cost = total;
+    // coset items, group level by group level (every item of an operator's group
+    // g - 1 precedes its group g: the waits' progress condition)
+    std::vector<Item> citems;
+    uint32_t const n_ops = uint32_t(ops.size());
+    {
+        uint32_t dep = n_ops;
+        for (CosetOp const &c : coset_ops)
+        {
+            ops[c.op].dep_base = dep;
+            dep += c.groups;
+        }
+        uint32_t max_groups = 0;
+        for (CosetOp const &c : coset_ops)
+            max_groups = std::max(max_groups, c.groups);
+        uint32_t const cpi = std::max<uint32_t>(sc.coset_tiles, 1);
+        for (uint32_t g = 0; g < max_groups; ++g)
+            for (CosetOp const &c : coset_ops)
+            {
+                if (g >= c.groups)
+                    continue;
+                BatchEntry const &e = b->entries[c.op];
+                fpb::BatchOp const &op = ops[c.op];
+                uint64_t const chunks = (e.n_states + (size_t(1) << op.lsb) - 1) >> op.lsb;
+                double const tile_bytes = double(size_t(1) << (op.d + op.lsb)) * (e.dtype == FP_C128 ? 16 : 8);
+                double const passes = (g == 0 && !e.accumulate) ? 2.0 : 3.0;
+                for (uint64_t y = 0; y < chunks; ++y)
+                    for (uint64_t q0 = 0; q0 < op.n_cosets; q0 += cpi)
+                    {
+                        uint64_t const cnt = std::min<uint64_t>(cpi, op.n_cosets - q0);
+                        citems.push_back({fpb::WorkItem{c.op, g, q0, cnt, uint32_t(y), 0}, 0.0,
+                                          double(cnt) * tile_bytes * passes});
+                    }
+            }
+    }
+    b->coset_items = citems.size();
+    // interleave the two lists by their shares of estimated bytes, so that small
+    // streaming items fill in while coset groups complete
+    std::vector<fpb::WorkItem> work;
+    work.reserve(items.size() + citems.size());
+    {
+        double ts = 0.0, tc = 0.0, cost = 0.0;
+        for (Item const &it : items)
+        {
+            ts += it.bytes;
+            cost += it.cost;
+        }
+        for (Item const &it : citems)
+            tc += it.bytes;
+        double ds = 0.0, dc = 0.0;
+        size_t is = 0, ic = 0;
+        while (is < items.size() || ic < citems.size())
+        {
+            bool const take_c =
+                ic < citems.size() && (is >= items.size() || dc / std::max(tc, 1.0) <= ds / std::max(ts, 1.0));
+            if (take_c)
+            {
+                dc += citems[ic].bytes;
+                work.push_back(citems[ic++].w);
+            }
+            else
+            {
+                ds += items[is].bytes;
+                work.push_back(items[is++].w);
+            }
+        }
+        b->est_cost = cost;
+    }
     if (work.size() > 0xffffffffu)
         return fail(FP_ERR_INVALID_ARGUMENT, "batch has too many tiles");
     b->n_items = uint32_t(work.size());
-    b->n_blocks = uint32_t(std::max<int>(1, device_cus() * int(std::max<uint32_t>(sc.blocks_per_cu, 1))));
+    b->n_done = n_ops + n_deps;
+    b->coset = !coset_ops.empty();
+    if (b->coset) // expectation items of the 512-thread kernel: twice the rows per column
+        for (fpb::BatchOp &op : ops)
+            if (op.kind == fpb::kExpect)
+            {
+                op.ct = uint32_t(std::min<uint64_t>(op.vpr, fpb::kTiledBlock));
+                op.rt = fpb::kTiledBlock / op.ct;
+            }
+    {
+        int const B = b->coset ? fpb::kTiledBlock : fpk::kBlock;
+        b->lds = std::max<size_t>(b->coset ? fpa::kTileBytes : 0, any_expect ? size_t(4) * B * sizeof(double) : 0);
+        if (b->lds > fprt::lds_bytes())
+            return fail(FP_ERR_UNSUPPORTED, "batch kernel needs %zu bytes of LDS per workgroup, the device has %zu",
+                        b->lds, fprt::lds_bytes());
+        static std::once_flag attr_once;
+        static hipError_t attr_err = hipSuccess;
+        std::call_once(attr_once, [] {
+            attr_err = hipFuncSetAttribute(reinterpret_cast<void const *>(&fpb::k_batch_persistent<fpb::kTiledBlock>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(fpa::kTileBytes));
+        });
+        if (attr_err != hipSuccess)
+            return fail(FP_ERR_RUNTIME, "hipFuncSetAttribute failed: %s", hipGetErrorString(attr_err));
+        int const per_cu = std::min<int>(int(std::max<uint32_t>(sc.blocks_per_cu, 1)), resident_blocks(b->coset, b->lds));
+        b->n_blocks = uint32_t(std::max<int>(1, device_cus() * per_cu));
+    }
 
     fp_status st;
     if ((st = ensure_capacity(&b->d_ops, b->cap_ops, ops.size(), stream)) != FP_OK ||
         (st = ensure_capacity(&b->d_items, b->cap_items, std::max<size_t>(work.size(), 1), stream)) != FP_OK ||
         (st = ensure_capacity(&b->d_partials, b->cap_partials, std::max<uint64_t>(partial_elems, 1), stream)) !=
             FP_OK ||
-        (st = ensure_capacity(&b->d_counter, b->cap_counter, ops.size() + 1, stream)) != FP_OK)
+        (st = ensure_capacity(&b->d_counter, b->cap_counter, size_t(b->n_done) + 2, stream)) != FP_OK)
         return st;
+    FP_HIP(hipMemsetAsync(b->d_counter + 1 + b->n_done, 0, sizeof(uint32_t), stream)); // error word
     b->h_ops = std::move(ops);
     b->h_items = std::move(work);
     FP_HIP(hipMemcpyAsync(b->d_ops, b->h_ops.data(), b->h_ops.size() * sizeof(fpb::BatchOp), hipMemcpyHostToDevice,
@@ -364,7 +572,8 @@ fp_status fp_hip_batch_set_schedule(fp_hip_batch *b, fp_hip_batch_schedule const
     if (!b || !s)
         return fail(FP_ERR_INVALID_ARGUMENT, "NULL batch or schedule");
     if (s->tile_vectors == 0 || s->ev_tile_work == 0 || s->blocks_per_cu == 0 || s->blocks_per_cu > 64 ||
-        (s->order != FP_BATCH_ORDER_LARGEST_FIRST && s->order != FP_BATCH_ORDER_AS_ADDED))
+        (s->order != FP_BATCH_ORDER_LARGEST_FIRST && s->order != FP_BATCH_ORDER_AS_ADDED) || s->coset_tiles == 0 ||
+        s->coset_tiles > 4096)
         return fail(FP_ERR_INVALID_ARGUMENT, "invalid batch schedule");
     if (std::memcmp(&b->sched, s, sizeof(*s)) != 0)
     {
@@ -393,6 +602,36 @@ fp_status fp_hip_batch_info(fp_hip_batch const *b, size_t *n_entries, size_t *n_
         *n_tiles = b->n_items;
     if (n_blocks)
         *n_blocks = b->n_blocks;
+    return FP_OK;
+}
+
+fp_status fp_hip_batch_coset_info(fp_hip_batch const *b, size_t *n_entries, size_t *n_groups, size_t *n_items,
+                                  double *plan_bytes)
+{
+    if (!b)
+        return fail(FP_ERR_INVALID_ARGUMENT, "batch is NULL");
+    if (n_entries)
+        *n_entries = b->coset_entries;
+    if (n_groups)
+        *n_groups = b->coset_groups;
+    if (n_items)
+        *n_items = b->coset_items;
+    if (plan_bytes)
+        *plan_bytes = b->plan_bytes;
+    return FP_OK;
+}
+
+fp_status fp_hip_batch_check(fp_hip_batch *b)
+{
+    if (!b)
+        return fail(FP_ERR_INVALID_ARGUMENT, "batch is NULL");
+    if (!b->built || !b->d_counter)
+        return FP_OK;
+    wait_idle(b);
+    uint32_t err = 0;
+    FP_HIP(hipMemcpy(&err, b->d_counter + 1 + b->n_done, sizeof(err), hipMemcpyDeviceToHost));
+    if (err != 0)
+        return fail(FP_ERR_RUNTIME, "a coset work item gave up waiting for the previous string group");
     return FP_OK;
 }
 
@@ -429,7 +668,7 @@ fp_status fp_hip_batch_launch(fp_hip_batch *b, void *stream)
 }
 
 // Coordinate descent over the schedule: tile size, expectation-tile work,
-// blocks per CU, then tile order; each candidate is timed over `reps` graph
+// blocks per CU, coset-item threshold and length, then tile order; each candidate is timed over `reps` graph
 // launches (after one warm-up) and the fastest is kept.  Outputs are
 // overwritten by the trial launches (accumulating entries accumulate reps+1
 // times), so callers tune on scratch outputs or re-initialise them.
@@ -489,7 +728,11 @@ fp_status fp_hip_batch_autotune(fp_hip_batch *b, void *stream, int reps, fp_hip_
             (r = try_field([](fp_hip_batch_schedule &s, uint64_t v) { s.ev_tile_work = v; },
                            {uint64_t(1) << 14, uint64_t(1) << 16, uint64_t(1) << 18, uint64_t(1) << 20})) == FP_OK &&
             (r = try_field([](fp_hip_batch_schedule &s, uint64_t v) { s.blocks_per_cu = uint32_t(v); },
-                           {1, 2, 4, 8})) == FP_OK)
+                           {1, 2, 4, 8})) == FP_OK &&
+            (r = try_field([](fp_hip_batch_schedule &s, uint64_t v) { s.coset_min_kib = uint32_t(v); },
+                           {0, 4096, 32768, 262144})) == FP_OK &&
+            (r = try_field([](fp_hip_batch_schedule &s, uint64_t v) { s.coset_tiles = uint32_t(v); },
+                           {4, 8, 16, 32})) == FP_OK)
             r = try_field([](fp_hip_batch_schedule &s, uint64_t v) { s.order = int(v); },
                           {FP_BATCH_ORDER_LARGEST_FIRST, FP_BATCH_ORDER_AS_ADDED});
     }

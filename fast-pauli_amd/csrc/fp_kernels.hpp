@@ -345,24 +345,24 @@ __device__ __forceinline__ void ev_rows(typename VecOf<T, NC>::type const *__res
 }
 
 // Block-wide sum over ty for every (tx, c); result valid in threads with
-// ty == 0.  `red` holds kBlock * NC values of type S.
-template <class S, int NC>
+// ty == 0.  `red` holds B * NC values of type S (B: threads per block).
+template <class S, int NC, int B = kBlock>
 __device__ __forceinline__ void reduce_over_rows(S (&v)[NC], S *red, uint32_t tx, uint32_t ty, uint32_t ct,
                                                  uint32_t rt)
 {
     uint32_t const tid = threadIdx.x;
 #pragma unroll
     for (int c = 0; c < NC; ++c)
-        red[c * kBlock + tid] = v[c];
+        red[c * B + tid] = v[c];
     __syncthreads();
     if (ty == 0 && tx < ct)
     {
 #pragma unroll
         for (int c = 0; c < NC; ++c)
         {
-            S s = red[c * kBlock + tx];
+            S s = red[c * B + tx];
             for (uint32_t r = 1; r < rt; ++r)
-                s += red[c * kBlock + r * ct + tx];
+                s += red[c * B + r * ct + tx];
             v[c] = s;
         }
     }

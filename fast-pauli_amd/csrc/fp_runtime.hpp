@@ -138,6 +138,8 @@ TileGeom tile_geom(int dtype, size_t n_qubits, size_t n_states, bool expect);
 fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, TileTables **out,
                       uint32_t wht_min = 0);
 fp_status tile_device(fp_hip_ops const *ops, TileTables *t, hipStream_t stream);
+// The tables were cut for `geo`, and one tile of `geo` fits `tile_bytes` of LDS.
+fp_status check_tile_fit(TileTables const *tt, TileGeom const &geo, int dtype, size_t tile_bytes);
 void invalidate_tiles(fp_hip_ops *ops);
 // Per-string EV matrix over cosets: for columns that do not fit in LDS whole.
 bool ev_coset_eligible(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_states);

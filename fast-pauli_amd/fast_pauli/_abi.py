@@ -131,7 +131,8 @@ def op_apply_plan(ops, dtype: int, dim: int, n_states: int, accumulate: bool = F
 class BatchSchedule(ctypes.Structure):
     """fp_hip_batch_schedule"""
     _fields_ = [("tile_vectors", ctypes.c_uint32), ("ev_tile_work", ctypes.c_uint64),
-                ("blocks_per_cu", ctypes.c_uint32), ("order", ctypes.c_int)]
+                ("blocks_per_cu", ctypes.c_uint32), ("order", ctypes.c_int),
+                ("coset_tiles", ctypes.c_uint32), ("coset_min_kib", ctypes.c_uint32)]
 
     def as_dict(self) -> dict:
         return {f: int(getattr(self, f)) for f, _ in self._fields_}
@@ -145,6 +146,9 @@ _SIGS.update({
     "fp_hip_batch_set_schedule": [_vp, ctypes.POINTER(BatchSchedule)],
     "fp_hip_batch_get_schedule": [_vp, ctypes.POINTER(BatchSchedule)],
     "fp_hip_batch_info": [_vp, ctypes.POINTER(_sz), ctypes.POINTER(_sz), ctypes.POINTER(_sz)],
+    "fp_hip_batch_coset_info": [_vp, ctypes.POINTER(_sz), ctypes.POINTER(_sz), ctypes.POINTER(_sz),
+                                ctypes.POINTER(ctypes.c_double)],
+    "fp_hip_batch_check": [_vp],
     "fp_hip_batch_launch": [_vp, _vp],
     "fp_hip_batch_set_graph": [_vp, _i],
     "fp_hip_batch_autotune": [_vp, _vp, _i, ctypes.POINTER(BatchSchedule), ctypes.POINTER(ctypes.c_float)],

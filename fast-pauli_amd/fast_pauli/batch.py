@@ -69,17 +69,30 @@ class Batch:
 
     @schedule.setter
     def schedule(self, d: dict) -> None:
-        s = _abi.BatchSchedule(**{k: int(v) for k, v in d.items()})
+        """Sets the given fields; the others keep their current values."""
+        s = _abi.BatchSchedule(**{k: int(v) for k, v in {**self.schedule, **d}.items()})
         _abi.call("fp_hip_batch_set_schedule", self.handle, ctypes.byref(s))
 
     def info(self) -> dict:
         a, b, c = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t()
         _abi.call("fp_hip_batch_info", self.handle, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
-        return {"entries": a.value, "tiles": b.value, "blocks": c.value}
+        e, g, i = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t()
+        pb = ctypes.c_double()
+        _abi.call("fp_hip_batch_coset_info", self.handle, ctypes.byref(e), ctypes.byref(g), ctypes.byref(i),
+                  ctypes.byref(pb))
+        return {"entries": a.value, "tiles": b.value, "blocks": c.value,
+                "coset_entries": e.value, "coset_groups": g.value, "coset_items": i.value,
+                "plan_bytes": pb.value}
+
+    def check(self) -> None:
+        """Waits for the last launch; raises if a coset item gave up waiting for its
+        operator's previous string group (an internal error)."""
+        _abi.call("fp_hip_batch_check", self.handle)
 
     def signature(self) -> str:
-        h = hashlib.sha1(json.dumps(sorted(self.entries)).encode()).hexdigest()
-        return h
+        # the schedule layout is part of the key: cached choices of another layout are not reused
+        key = {"entries": sorted(self.entries), "schedule": [f for f, _ in _abi.BatchSchedule._fields_]}
+        return hashlib.sha1(json.dumps(key).encode()).hexdigest()
 
     def autotune(self, stream=None, reps: int = 3, cache_path: str | None = None,
                  use_cache: bool = True) -> dict:
@@ -94,8 +107,8 @@ class Batch:
                 except (OSError, ValueError):
                     cached = None
             if cached:
-                self.schedule = cached["schedule"]
-                return {"schedule": cached["schedule"], "ms": cached["ms"], "cached": True}
+                self.schedule = {**self.schedule, **cached["schedule"]}
+                return {"schedule": self.schedule, "ms": cached["ms"], "cached": True}
         chosen = _abi.BatchSchedule()
         ms = ctypes.c_float()
         _abi.call("fp_hip_batch_autotune", self.handle, _abi._handle(stream), int(reps), ctypes.byref(chosen),

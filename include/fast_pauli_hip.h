@@ -304,6 +304,10 @@ typedef struct fp_hip_batch_schedule
     uint64_t ev_tile_work;  /* strings x pairs x vectors-per-row per expectation tile */
     uint32_t blocks_per_cu; /* persistent blocks per compute unit */
     int order;              /* FP_BATCH_ORDER_* */
+    uint32_t coset_tiles;   /* coset apply items: 64 KiB tiles (cosets) per item, >= 1 */
+    uint32_t coset_min_kib; /* apply entries with states of at least this many KiB run as
+                               coset items (string groups sharing LDS tiles) when that moves
+                               fewer bytes; 0: never */
 } fp_hip_batch_schedule;
 
 fp_status fp_hip_batch_create(fp_hip_batch **batch);
@@ -314,6 +318,16 @@ fp_status fp_hip_batch_add(fp_hip_batch *batch, int kind, fp_hip_ops const *ops,
 fp_status fp_hip_batch_set_schedule(fp_hip_batch *batch, fp_hip_batch_schedule const *schedule);
 fp_status fp_hip_batch_get_schedule(fp_hip_batch const *batch, fp_hip_batch_schedule *schedule);
 fp_status fp_hip_batch_info(fp_hip_batch const *batch, size_t *n_entries, size_t *n_tiles, size_t *n_blocks);
+/* Of the last build: entries run as coset items, their string groups and items, and the
+ * plan's state/output bytes per launch (coset entries: 3 per string group, less the first
+ * group's output read unless accumulating; streaming apply: strings + 1 (+ 1 accumulating);
+ * expectation: one state pass per string). */
+fp_status fp_hip_batch_coset_info(fp_hip_batch const *batch, size_t *n_entries, size_t *n_groups, size_t *n_items,
+                                  double *plan_bytes);
+/* Waits for the batch's last launch; FP_ERR_RUNTIME if a coset item of any launch since the
+ * last build gave up waiting for the previous string group (an internal error: its outputs
+ * are then unspecified). */
+fp_status fp_hip_batch_check(fp_hip_batch *batch);
 fp_status fp_hip_batch_launch(fp_hip_batch *batch, void *stream);
 /* enabled = 1 (default): launches replay a captured hipGraph; 0: kernels are enqueued
  * directly (cheaper when every launch has new entries, as in a serving loop). */

@@ -114,7 +114,7 @@ def test_schedules_do_not_change_results(hip):
     for sched in ({"tile_vectors": 512, "ev_tile_work": 1 << 14, "blocks_per_cu": 1, "order": 1},
                   {"tile_vectors": 8192, "ev_tile_work": 1 << 20, "blocks_per_cu": 8, "order": 0}):
         b.schedule = sched
-        assert b.schedule == sched
+        assert {k: b.schedule[k] for k in sched} == sched
         for e in entries:
             e.reset()
         b.launch()
@@ -131,6 +131,8 @@ def test_schedules_do_not_change_results(hip):
                     assert H.rel_err(r - e.base, r0 - e.base) <= tol
     with pytest.raises(ValueError):
         b.schedule = {"tile_vectors": 0, "ev_tile_work": 1, "blocks_per_cu": 1, "order": 0}
+    with pytest.raises(ValueError):
+        b.schedule = {"coset_tiles": 0}
 
 
 def test_overlapping_outputs_rejected(hip):
@@ -164,6 +166,7 @@ def test_uneven_workload_autotuned_bit_exact(hip, tmp_path):
     assert res["ms"] > 0
     b.launch()
     _abi.synchronize()
+    b.check()
     d_ref = _abi.DeviceBuffer(max((1 << n) * S * 8 for _, _, _, n, S in items))
     for ops, d_in, d_out, n, S in items:
         _abi.call("fp_hip_op_apply", ops.handle, C64, d_in.ptr, d_ref.ptr, 1 << n, S, 0, None)
@@ -214,4 +217,62 @@ def test_batch_rebuilds_after_ops_reassigned(hip, graph):
     ops.assign(syn.random_codes(4, n + 1, seed=4))
     with pytest.raises(ValueError):
         b.launch()
+    b.destroy()
+
+
+def coset_entries():
+    """Apply entries large enough (with a 1 KiB threshold) and with few enough string
+    groups to run as coset items: power-of-two and other state counts, one and
+    several state chunks, complex64 and complex128, overwrite and accumulate; plus
+    streaming apply and expectation entries in the same 512-thread kernel."""
+    A, E = _abi.FP_BATCH_APPLY, _abi.FP_BATCH_EXPECT
+    specs = [
+        (A, 12, 40, 3, C64, False), (A, 13, 64, 59, C64, False), (A, 12, 48, 64, C64, True),
+        (A, 12, 30, 16, C128, False), (A, 12, 33, 5, C128, True), (A, 11, 24, 8, C64, False),
+        (E, 10, 20, 7, C64, False), (A, 9, 12, 4, C64, False), (E, 12, 30, 16, C128, True),
+        (A, 12, 6, 1, C64, False),
+    ]
+    return [Entry(*s, seed=40 + i) for i, s in enumerate(specs)]
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_coset_items_bit_exact(hip, graph):
+    """Large apply entries run as coset items (string groups sharing 64 KiB LDS
+    tiles, groups of one operator in order): IEEE-equal to the operator alone,
+    for every coset-item length, over repeated launches."""
+    H.say(f"batch coset items graph={graph}")
+    entries = coset_entries()
+    b = Batch()
+    if not graph:
+        _abi.call("fp_hip_batch_set_graph", b.handle, 0)
+    for e in entries:
+        b.add(e.kind, e.ops, e.d_in.ptr, e.d_out.ptr, 1 << e.n, e.S, e.dtype, e.accumulate)
+    refs = [e.alone() for e in entries]
+    for tiles in (1, 3, 16):
+        b.schedule = {"coset_min_kib": 1, "coset_tiles": tiles}
+        for rep in range(3):
+            for e in entries:
+                e.reset()
+            b.launch()
+            _abi.synchronize()
+            info = b.info()
+            assert info["coset_entries"] == 6 and info["coset_items"] > 0, info
+            for e, r in zip(entries, refs):
+                got = e.result()
+                if e.kind == _abi.FP_BATCH_APPLY:
+                    assert np.array_equal(got, r), (tiles, rep, e.n, e.N, e.S, e.dtype, e.accumulate)
+                else:
+                    tol = 1e-12 if e.dtype == C128 else 1e-5
+                    assert H.rel_err(got - e.base, r - e.base) <= tol
+            b.check()
+    # threshold 0: no coset items, same results
+    b.schedule = {"coset_min_kib": 0}
+    for e in entries:
+        e.reset()
+    b.launch()
+    _abi.synchronize()
+    assert b.info()["coset_entries"] == 0
+    for e, r in zip(entries, refs):
+        if e.kind == _abi.FP_BATCH_APPLY:
+            assert np.array_equal(e.result(), r)
     b.destroy()
