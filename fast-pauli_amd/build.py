@@ -52,10 +52,17 @@ def build_hip() -> str:
 
     srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
     objs = [os.path.join(CSRC, os.path.basename(src)[:-4] + ".o") for src in srcs]
-    jobs = max(1, min(len(srcs), int(os.environ.get("MAX_JOBS", "0") or 0) or (os.cpu_count() or 1)))
+    # plain C++ units (no HIP): the host compiler
+    cpp_srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".cpp"))
+    cpp_objs = [os.path.join(CSRC, os.path.basename(src)[:-4] + ".host.o") for src in cpp_srcs]
+    cxx = os.environ.get("CXX", "g++")
+    cmds = [[HIPCC, *HIP_FLAGS, "-c", f"-I{INCLUDE}", f"-I{CSRC}", "-o", o, src] for src, o in zip(srcs, objs)]
+    cmds += [[cxx, "-O3", "-std=c++20", "-fPIC", "-Wall", "-Wextra", "-c", f"-I{INCLUDE}", f"-I{CSRC}", "-o", o, src]
+             for src, o in zip(cpp_srcs, cpp_objs)]
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "0") or 0) or (os.cpu_count() or 1)))
     with ThreadPoolExecutor(max_workers=jobs) as pool:
-        list(pool.map(lambda so: _run([HIPCC, *HIP_FLAGS, "-c", f"-I{INCLUDE}", f"-I{CSRC}", "-o", so[1], so[0]]),
-                      zip(srcs, objs)))
+        list(pool.map(_run, cmds))
+    objs += cpp_objs
     tmp = LIB_HIP + ".tmp"
     _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *objs])
     os.replace(tmp, LIB_HIP)
@@ -98,9 +105,27 @@ def build_cpp_test() -> str | None:
     return out
 
 
+# address and undefined-behaviour sanitizers for the host-only table-builder test
+SANITIZE = ["-fsanitize=address,undefined", "-fno-sanitize-recover=all", "-fno-omit-frame-pointer"]
+
+
+def build_tile_test() -> str | None:
+    """The host tile-table builder's unit test (tests/cpp/test_tile_tables.cpp), built
+    from the plain C++ builder sources with the sanitizers; needs no GPU."""
+    src = os.path.join(ROOT, "tests", "cpp", "test_tile_tables.cpp")
+    if not os.path.exists(src):
+        return None
+    out = os.path.join(ROOT, "tests", "cpp", "test_tile_tables")
+    cxx = os.environ.get("CXX", "g++")
+    _run([cxx, "-O1", "-g", "-std=c++20", "-Wall", "-Wextra", *SANITIZE, f"-I{INCLUDE}", f"-I{CSRC}",
+          "-o", out + ".tmp", src, os.path.join(CSRC, "fp_tiles_host.cpp"), os.path.join(CSRC, "fp_error.cpp")])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all() -> list[str]:
     built = [build_hip()]
-    for extra in (build_python(), build_cpp_test()):
+    for extra in (build_python(), build_cpp_test(), build_tile_test()):
         if extra:
             built.append(extra)
     return built
@@ -114,5 +139,7 @@ if __name__ == "__main__":
         build_python()
     elif what == "cpptest":
         build_cpp_test()
+    elif what == "tiletest":
+        build_tile_test()
     else:
         build_all()

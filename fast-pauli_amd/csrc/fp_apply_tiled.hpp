@@ -39,18 +39,6 @@ namespace fpa
 constexpr int kThreads = 512;
 constexpr uint32_t kTileBytes = 64u * 1024u;
 
-struct ApplyStr
-{
-    uint64_t zc;  // z compressed onto the coset-index bits: coset sign = parity(q & zc)
-    uint32_t A;   // local x mask
-    uint32_t Z;   // local z mask
-    double cr;    // folded coefficient c_s (-i)^{nY_s}, complex128
-    double ci;
-    float fr;     // the same, complex64 (rounded as the streaming kernel's table)
-    float fi;
-    uint32_t pad[2];
-};
-
 // One group: launch over cosets [0, n_cosets) of group g, blocks striding by
 // gridDim.x; state chunk blockIdx.y.  PAIRED requires complex64, S_blk >= 2, a
 // power-of-two state count that S_blk divides, and 2^d rows >= one full tile.

@@ -34,52 +34,13 @@
 #include <stdint.h>
 
 #include "fp_ev_tiled.hpp"
+#include "fp_tile_types.hpp"
 #include "fp_kernels.hpp"
 
 namespace fpl
 {
 
-constexpr int kChunk = 8;
 constexpr int kThreads = 1024;
-
-// A class of strings with equal x and equal half of w is evaluated by one
-// Walsh-Hadamard transform of G over the leader rows when it holds at least
-// kWhtMin strings (split into classes of at most kWhtMax):
-//     ev_s = scale_s * sum_leaders (-1)^{parity(a & Z_s)} G(a) = scale_s * W[compress(Z_s)],
-// one pass over the rows and (d-1) butterfly levels instead of one pass per string.
-// One wave transforms a class on its own, in parts of kWhtPartBytes of G held in
-// a private LDS slice (no block barrier); the parts are combined per string by
-// their top index bits.
-constexpr uint32_t kWhtMin = 12;
-constexpr uint32_t kWhtMax = 256;        // 4 strings per lane
-constexpr uint32_t kWhtMinRowsLog2 = 10; // columns shorter than this keep the per-string loop
-constexpr uint32_t kWhtPartBytes = 2048;
-constexpr uint32_t kWhtWaves = 15;       // waves with a slice (the last wave takes per-string chunks only)
-constexpr uint32_t kWhtLdsBytes = kWhtWaves * kWhtPartBytes;
-
-enum : uint32_t
-{
-    kAllRows = 0,
-    kPairs = 1,
-    kPairsLane = 2
-};
-
-struct EvChunk
-{
-    uint32_t x;     // x mask (n <= 14: fits 32 bits)
-    uint32_t h;     // highest set bit of x (pair modes)
-    uint32_t mode;  // kAllRows / kPairs / kPairsLane
-    uint32_t count; // strings in this chunk (1..kChunk)
-    uint32_t first; // index of the first EvStr
-    uint32_t imag;  // 1: every string of the chunk takes Im w (odd nY), 0: Re w
-    uint32_t pad[2];
-};
-
-// Mode of a string with x mask x (highest set bit h) over `rows` rows.
-__host__ __device__ constexpr uint32_t ev_mode(uint64_t x, uint32_t h, uint64_t rows)
-{
-    return x == 0 ? kAllRows : (h >= 6 ? kPairs : (rows >= 128 ? kPairsLane : kAllRows));
-}
 
 // Row offset of lane `lane` within a pair of 64-row blocks (kPairsLane), or
 // within one block (other modes); the lane's sign bit is parity(offset & z & 127).
@@ -88,14 +49,6 @@ __device__ __forceinline__ uint32_t lane_row(uint32_t mode, uint32_t h, uint32_t
     return mode == kPairsLane ? lane + (((lane >> h) & 1u) << 6) : lane;
 }
 
-struct EvStr
-{
-    uint32_t z;
-    uint32_t imag; // 1: G = Im w, 0: G = Re w
-    uint32_t out;  // output row (original string index)
-    uint32_t zc;   // coset kernel: z compressed onto the coset-index bits (0 otherwise)
-    double scale;  // eps (kAllRows) or 2 eps (kPairs)
-};
 
 // v with bit 0 of `bits` inserted as a zero at position h
 __device__ __forceinline__ uint32_t insert_zero(uint32_t v, uint32_t h)

@@ -45,6 +45,7 @@
 #include <type_traits>
 
 #include "fp_kernels.hpp"
+#include "fp_tile_types.hpp"
 
 namespace fpt
 {
@@ -55,82 +56,6 @@ template <int TKB> struct TileCfg
     static constexpr uint32_t kBytes = uint32_t(TKB) * 1024u;
     static constexpr int kThreads = TKB * 8;
 };
-constexpr int kMaxDh = 13; // string-span dimensions per tile (B table of 2^13 entries, read once per block)
-
-struct TileGroup
-{
-    uint32_t n_chunks;
-    uint32_t chunk_first;
-    uint32_t n_piv;    // dh: pivots (high dimensions of V)
-    uint32_t btab_off; // first entry of this group's B table (2^dh entries)
-    uint64_t piv_lo;   // pivot bit positions 0..7, ascending, 8 bits each
-    uint64_t piv_hi;   // pivot bit positions 8..15
-    uint32_t hoist;     // bit hoist_bit(W, U1, r): every wave of a block has at most one
-                        // unit and it takes the one-string fast path (expectation tables)
-    uint32_t shared;    // bit shared_bit(W, c64, rr): the group runs the shared-leader
-                        // form (expectation tables)
-    uint32_t str_first; // first ShStr of the group (expectation tables)
-    uint32_t n_str;     // strings of the group
-    uint32_t n_wht;     // per-string EV matrix tables: the first n_wht chunks are transform classes
-    uint32_t pad;
-};
-
-// Strings of a shared-leader group: every local x mask has bit k set, so the
-// leader rows (local bit k clear) are common to all of them.  At most
-// max_shared(W) strings for the instance with W waves per block (its per-string
-// registers), kMaxShared over all instances.
-constexpr int kMaxShared = 10;
-__host__ __device__ constexpr int max_shared(uint32_t W)
-{
-    return W == 16 ? 10 : 9;
-}
-
-struct ShStr
-{
-    uint64_t zc;   // z compressed onto the coset-index bits: coset sign = parity(q & zc)
-    uint32_t A;    // local x mask (bit k set)
-    uint32_t Z;    // local z mask
-    uint32_t imag; // 1: Im w (odd nY), 0: Re w
-    uint32_t pad;
-    double sre; // scale c_s * kappa_s
-    double sim;
-};
-
-// Bit of TileGroup::shared for W waves per block, complex64 (two states per
-// lane) or not, and 2^rr leader rows per pass.
-__host__ __device__ constexpr uint32_t shared_bit(uint32_t W, bool c64, uint32_t rr)
-{
-    return (W == 16 ? 16u : 0u) + (c64 ? 8u : 0u) + rr;
-}
-
-// Bit of TileGroup::hoist for W waves per block, U1 passes per fast-path step and
-// 2^r leader rows per pass (r = 6 - log2 S_blk).
-__host__ __device__ constexpr uint32_t hoist_bit(uint32_t W, uint32_t U1, uint32_t r)
-{
-    return (W == 16 ? 16u : 0u) + (U1 == 8 ? 8u : 0u) + r;
-}
-
-struct TileChunk
-{
-    uint32_t A;     // local x mask; 0: every row, kappa 1
-    uint32_t h;     // top bit of A
-    uint32_t count; // strings in this chunk (1..kChunk), all with local mask A
-    uint32_t first; // first TileStr
-    uint32_t need;  // bit 0: some string uses Re w, bit 1: some string uses Im w
-    uint32_t pad[3];
-};
-
-struct TileStr
-{
-    uint64_t zc;   // z compressed onto the coset-index bits: coset sign = parity(q & zc)
-    uint32_t zt;   // local z with bit h removed: row sign of leader p = parity(p & zt)
-    uint32_t imag; // 1: Im w (odd nY), 0: Re w
-    double sre;    // scale c_s * kappa_s
-    double sim;
-};
-
-constexpr int kChunk = 4;
-
 __device__ __forceinline__ uint32_t insert_zero_bit32(uint32_t v, uint32_t b)
 {
     uint32_t const lo = v & ((1u << b) - 1u);

@@ -27,19 +27,6 @@
 namespace fprt
 {
 
-thread_local std::string g_last_error;
-
-fp_status fail(fp_status code, char const *fmt, ...)
-{
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof(buf), fmt, ap);
-    va_end(ap);
-    g_last_error = buf;
-    return code;
-}
-
 // ---------------------------------------------------------------------------
 // Per-stream scratch: reduction partials etc.  Grown on demand (after the
 // stream has drained), never shrunk.  Calls on one stream are ordered, and the
@@ -816,17 +803,6 @@ extern "C"
 const char *fp_hip_version(void)
 {
     return "fast_pauli_hip 0.1 (gfx950)";
-}
-
-size_t fp_hip_last_error(char *buf, size_t len)
-{
-    if (buf && len)
-    {
-        size_t const n = std::min(len - 1, g_last_error.size());
-        std::memcpy(buf, g_last_error.data(), n);
-        buf[n] = 0;
-    }
-    return g_last_error.size();
 }
 
 fp_status fp_hip_device_count(int *count)

@@ -17,12 +17,10 @@
 #include "fp_apply_tiled.hpp"
 #include "fp_ev_tiled.hpp"
 #include "fp_kernels.hpp"
+#include "fp_tiles_host.hpp"
 
 namespace fprt
 {
-
-// Records a printf-style message as the calling thread's last error and returns `code`.
-fp_status fail(fp_status code, char const *fmt, ...) __attribute__((format(printf, 2, 3)));
 
 // Per-stream device scratch of at least `bytes`; the pointer stays valid until
 // the next, larger request on the same stream.  Callers hold a StreamGuard for
@@ -94,21 +92,6 @@ struct VecGeom
 
 VecGeom vec_geom(int dtype, size_t dim, size_t n_states, void const *a, void const *b);
 
-// Coset-tile geometry of the tiled expectation kernel: 2^lsb states per block,
-// tiles of 2^d rows whose low k bits are consecutive rows and whose dh = d - k
-// high dimensions come from the string group's span.
-struct TileGeom
-{
-    uint32_t lsb, k, d, dh, tile_kb;
-};
-
-enum
-{
-    kTilesOpExpect = 0, // fpt::TileChunk / fpt::TileStr records (PauliOp expectation)
-    kTilesEvMatrix = 1, // fpl::EvChunk / fpl::EvStr records (per-string EV matrix on cosets)
-    kTilesOpApply = 2   // fpa::ApplyStr records, groups of consecutive strings (PauliOp apply)
-};
-
 // String-group tables of one kind and geometry (k, dh), cached per operator
 // set: a host image (groups | B tables | chunks | strings) and its device copy.
 // Re-targeting the operator set invalidates them but keeps both buffers for
@@ -132,8 +115,6 @@ struct TileTables
     fpt::ShStr *sh = nullptr; // shared-leader string records (expectation tables)
 };
 
-// `expect`: tiles for the expectation kernel (complex128 takes 128-byte row slices), else apply
-TileGeom tile_geom(int dtype, size_t n_qubits, size_t n_states, bool expect);
 // `wht_min`: for kTilesEvMatrix, the smallest class evaluated through a transform (part of the key)
 fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, TileTables **out,
                       uint32_t wht_min = 0);
