@@ -86,8 +86,20 @@ __device__ void wht_class(typename fpk::VecOf<T, 1>::type const *col, uint32_t d
     uint32_t const L = x ? d - 1 : d;
     uint32_t const gb = L < kPartLog2 ? L : kPartLog2;
     uint32_t const gsize = 1u << gb, gmask = gsize - 1u, pb = L - gb;
+    // FP32 slice layout: index i at i ^ (h << 3) ^ h, h = i >> 6 (a bijection of
+    // [0, 512) that keeps bits 6-8).  In every butterfly round, and in the stores of
+    // G, the 64 lanes then touch 64 different banks for each of their 8 values
+    // (with the plain layout 8 lanes share a bank in the rounds over bits 0-2 and 3-5).
+    // The map is linear over XOR, so sw(a | b) = sw(a) ^ sw(b) for disjoint bits: a
+    // lane part once per round and a wave-uniform part per value.
+    auto sw = [](uint32_t i) -> uint32_t {
+        if constexpr (kVals == 8)
+            return i ^ (((i >> 6) & 7u) * 9u);
+        else
+            return i;
+    };
     A res[kRes];
-    uint32_t zc[kRes];
+    uint32_t zc[kRes], zs[kRes]; // compressed z; its in-part index in the slice layout
 #pragma unroll
     for (int j = 0; j < kRes; ++j)
     {
@@ -95,7 +107,9 @@ __device__ void wht_class(typename fpk::VecOf<T, 1>::type const *col, uint32_t d
         uint32_t const s = lane + 64u * uint32_t(j);
         uint32_t const z = s < ch.count ? strs[ch.first + s].z : 0u;
         zc[j] = x ? ((z & ((1u << h) - 1u)) | ((z >> (h + 1)) << h)) : z;
+        zs[j] = sw(zc[j] & gmask);
     }
+    uint32_t const sw_lane = sw(lane);
     for (uint32_t part = 0; part < (1u << pb); ++part)
     {
         // G of this part's leaders into the slice
@@ -118,7 +132,7 @@ __device__ void wht_class(typename fpk::VecOf<T, 1>::type const *col, uint32_t d
                     A const br = A(w[0]), bi = A(w[1]);
                     g = ch.imag ? fma(ar, bi, -(ai * br)) : fma(ar, br, ai * bi);
                 }
-                scr[i] = g;
+                scr[sw_lane ^ sw(64u * m)] = g;
             }
         }
         // W[z] = sum_i (-1)^{parity(i & z)} G[i]: rounds of kRb index bits in registers
@@ -131,10 +145,11 @@ __device__ void wht_class(typename fpk::VecOf<T, 1>::type const *col, uint32_t d
             {
                 // group base: the group index with rb zero bits inserted at b0
                 uint32_t const base = ((grp >> b0) << (b0 + rb)) | (grp & ((1u << b0) - 1u));
+                uint32_t const sbase = sw(base);
                 A v[kVals];
 #pragma unroll
                 for (uint32_t m = 0; m < kVals; ++m)
-                    v[m] = m < (1u << rb) ? scr[base | (m << b0)] : A(0);
+                    v[m] = m < (1u << rb) ? scr[sbase ^ sw(m << b0)] : A(0);
 #pragma unroll
                 for (uint32_t lvl = 0; lvl < kRb; ++lvl)
 #pragma unroll
@@ -148,7 +163,7 @@ __device__ void wht_class(typename fpk::VecOf<T, 1>::type const *col, uint32_t d
 #pragma unroll
                 for (uint32_t m = 0; m < kVals; ++m)
                     if (m < (1u << rb))
-                        scr[base | (m << b0)] = v[m];
+                        scr[sbase ^ sw(m << b0)] = v[m];
             }
         }
         wave_lds_sync();
@@ -158,7 +173,7 @@ __device__ void wht_class(typename fpk::VecOf<T, 1>::type const *col, uint32_t d
             uint32_t const s = lane + 64u * uint32_t(j);
             if (s < ch.count)
             {
-                A const t = scr[zc[j] & gmask];
+                A const t = scr[zs[j]];
                 res[j] += (__popc((zc[j] >> gb) & part) & 1) ? -t : t;
             }
         }
