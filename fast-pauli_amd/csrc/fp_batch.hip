@@ -174,6 +174,13 @@ int resident_blocks(bool coset, size_t lds)
     return (e == hipSuccess && nb > 0) ? nb : 1;
 }
 
+// Coset items of complex64 entries with an even state count take two states per
+// slot (16-byte loads, stores and LDS reads).
+bool coset_paired(BatchEntry const &e, fprt::TileGeom const &g)
+{
+    return e.dtype == FP_C64 && e.n_states % 2 == 0 && g.lsb >= 1;
+}
+
 // Coset geometry and string-group tables of an apply entry, when its state is at
 // least `min_kib` KiB and its groups move fewer bytes than the streaming form:
 // (3 G - 1) state passes (2 G with no output read for the first group, plus a
@@ -188,9 +195,10 @@ fp_status coset_plan(BatchEntry const &e, uint32_t min_kib, fprt::TileGeom *geo,
         return FP_OK;
     fprt::TileGeom const g = fprt::tile_geom(e.dtype, e.ops->n_qubits, e.n_states, false);
     // the kernel's slot decomposition needs the low k row bits inside one thread's
-    // 2^(9 - lsb) consecutive local rows (rm >= k)
+    // 2^(9 - lsbx) consecutive local rows (rm >= k); lsbx = lsb - 1 with two states per slot
     uint32_t const log2_block = uint32_t(__builtin_ctz(uint32_t(fpb::kTiledBlock)));
-    if (g.dh == 0 || g.d >= e.ops->n_qubits || g.lsb > log2_block || log2_block - g.lsb < g.k)
+    uint32_t const lsbx = g.lsb - (coset_paired(e, g) ? 1u : 0u);
+    if (g.dh == 0 || g.d >= e.ops->n_qubits || lsbx > log2_block || log2_block - lsbx < g.k)
         return FP_OK;
     fprt::TileTables *t = nullptr;
     fp_status s = fprt::tile_tables(e.ops, fprt::kTilesOpApply, g.k, g.dh, &t);
@@ -277,6 +285,7 @@ fp_status build(fp_hip_batch *b, hipStream_t stream)
             if (us != FP_OK)
                 return us;
             op.coset = 1;
+            op.nc = coset_paired(e, geo) ? 2u : 1u;
             op.groups = tt->groups;
             op.btab = tt->btab;
             op.astrs = static_cast<fpa::ApplyStr const *>(tt->strs);

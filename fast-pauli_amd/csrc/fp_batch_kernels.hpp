@@ -79,7 +79,7 @@ struct BatchOp
     uint64_t const *btab;
     fpa::ApplyStr const *astrs;
     uint64_t n_cosets;               // cosets of 2^d rows
-    uint32_t coset;                  // 1: this operator runs as coset items
+    uint32_t coset;                  // 1: this operator runs as coset items (nc: states per slot)
     uint32_t lsb, k, d;              // tile: 2^d rows (the low k row bits among them) x 2^lsb states
     uint32_t sshift;                 // log2(n_states) when a power of two, else 64
     uint32_t dep_base;               // group g's completion counter: deps[dep_base + g]
@@ -169,9 +169,12 @@ __device__ void apply_tile(BatchOp const &op, uint64_t begin, uint64_t count)
 }
 
 // Coset apply item: cosets [q0, q0 + count) of string group g, states
-// [chunk << lsb, (chunk + 1) << lsb).  Slot m of thread tid is tile element
-// tid + m * B: local row a_m = a_lane + (m << rm) (a_lane = tid >> lsb < 2^rm,
-// disjoint bits) and state tx = tid & (2^lsb - 1).  The local-to-global row map
+// [chunk << lsb, (chunk + 1) << lsb).  A slot holds P states of one row (P = 2:
+// complex64 with an even state count, two neighbouring states moved by 16-byte
+// loads, stores and LDS reads); a row of the chunk has 2^lsbx slots, lsbx =
+// lsb - log2 P.  Slot m of thread tid is tile slot tid + m * B: local row
+// a_m = a_lane + (m << rm) (a_lane = tid >> lsbx < 2^rm, disjoint bits) and
+// states P tx .. P tx + P - 1, tx = tid & (2^lsbx - 1).  The local-to-global row map
 // (low k bits kept, the rest through the group's table bt of span combinations)
 // is linear over XOR, and rm >= k, so the row of slot m in the coset with base
 // cb is cb ^ roff(a_lane) ^ u_m with u_m = bt[(m << rm) >> k] the same in every
@@ -181,11 +184,12 @@ __device__ void apply_tile(BatchOp const &op, uint64_t begin, uint64_t count)
 // folded c_s (-i)^{nY_s}.  Each output element sees out + term_0 + term_1 + ...
 // in the operator's string order (cmul_exact, add_exact), as in the
 // single-operator kernels.  Requires 2^n * S < 2^32 (32-bit element offsets).
-template <class T, int B>
+template <class T, int B, int P>
 __device__ void apply_coset_item(BatchOp const &op, uint32_t g, uint32_t chunk, uint64_t q0, uint64_t count,
                                  unsigned char *smem)
 {
-    typedef typename fpk::VecOf<T, 1>::type V;
+    static_assert(P == 1 || (P == 2 && sizeof(T) == 4), "two states per slot: complex64 only");
+    typedef typename fpk::VecOf<T, P>::type V; // one slot
     constexpr uint32_t kSlots = fpa::kTileBytes / sizeof(V);
     constexpr uint32_t EPT = kSlots / B;
     constexpr uint32_t kLog2B = B == 512 ? 9u : (B == 256 ? 8u : 10u);
@@ -201,15 +205,16 @@ __device__ void apply_coset_item(BatchOp const &op, uint32_t g, uint32_t chunk, 
     uint32_t const lsb = op.lsb, k = op.k, d = op.d, sshift = op.sshift;
     uint32_t const n_states = uint32_t(op.n_states);
     uint32_t const tid = threadIdx.x;
-    uint32_t const tx = tid & ((1u << lsb) - 1u);
-    uint32_t const st0 = (chunk << lsb) + tx; // this thread's state
-    bool const t_live = st0 < n_states;
-    uint32_t const tile_slots = (1u << d) << lsb;
+    uint32_t const lsbx = P == 2 ? lsb - 1u : lsb;
+    uint32_t const tx = tid & ((1u << lsbx) - 1u);
+    uint32_t const st0 = (chunk << lsb) + uint32_t(P) * tx; // this thread's first state
+    bool const t_live = st0 < n_states;                    // P = 2: n_states is even
+    uint32_t const tile_slots = (1u << d) << lsbx;
     bool const full = tile_slots == kSlots && (chunk << lsb) + (1u << lsb) <= n_states;
     bool const s_pow2 = sshift < 64u;
     bool const init_zero = g == 0 && !op.accumulate;
-    uint32_t const rm = kLog2B - lsb;
-    uint32_t const a_lane = tid >> lsb;
+    uint32_t const rm = kLog2B - lsbx;
+    uint32_t const a_lane = tid >> lsbx;
     uint32_t const lane_mask = (1u << rm) - 1u;
     uint32_t const roff = a_lane < (1u << d) ? uint32_t(bt[a_lane >> k] ^ (a_lane & ((1u << k) - 1u))) : 0u;
     uint32_t u[EPT];
@@ -217,10 +222,14 @@ __device__ void apply_coset_item(BatchOp const &op, uint32_t g, uint32_t chunk, 
     for (uint32_t m = 0; m < EPT; ++m)
         u[m] = (m << rm) < (1u << d) ? uint32_t(bt[(m << rm) >> k]) : 0u;
     auto slot_live = [&](uint32_t m) { return full || (tid + m * B < tile_slots && t_live); };
-    // element index of slot m in the coset whose base, xor this lane's offset, is cbl
+    // slot index (in units of P states) of slot m in the coset whose base, xor this
+    // lane's offset, is cbl
     auto index_of = [&](uint32_t cbl, uint32_t m) -> uint32_t {
         uint32_t const row = cbl ^ u[m];
-        return s_pow2 ? ((row << sshift) | st0) : row * n_states + st0;
+        if constexpr (P == 2)
+            return s_pow2 ? ((row << (sshift - 1u)) | (st0 >> 1)) : row * (n_states >> 1) + (st0 >> 1);
+        else
+            return s_pow2 ? ((row << sshift) | st0) : row * n_states + st0;
     };
 
     V reg[EPT];
@@ -280,7 +289,7 @@ __device__ void apply_coset_item(BatchOp const &op, uint32_t g, uint32_t chunk, 
                 ci = st.fi;
             }
             uint32_t const A_lane = st.A & lane_mask, A_m = st.A & ~lane_mask;
-            V const *pl = tile + (((a_lane ^ A_lane) << lsb) | tx);
+            V const *pl = tile + (((a_lane ^ A_lane) << lsbx) | tx);
             bool const lneg = ((uint32_t(__popc(a_lane & st.Z)) ^ cs) & 1u) != 0;
             T const lcr = lneg ? -cr : cr;
             T const lci = lneg ? -ci : ci;
@@ -288,14 +297,18 @@ __device__ void apply_coset_item(BatchOp const &op, uint32_t g, uint32_t chunk, 
             for (uint32_t m = 0; m < EPT; ++m)
             {
                 uint32_t const mrow = m << rm;
-                V const v = pl[(mrow ^ A_m) << lsb];
+                V const v = pl[(mrow ^ A_m) << lsbx];
                 uint32_t const mbit = uint32_t(__popc(mrow & st.Z) & 1u) << 31;
                 T const a = fpt::xor_sign_bit(lcr, mbit);
                 T const b = fpt::xor_sign_bit(lci, mbit);
-                T orr, oi;
-                fpk::cmul_exact<T>(a, b, v[0], v[1], orr, oi);
-                acc[m][0] = fpk::add_exact<T>(acc[m][0], orr);
-                acc[m][1] = fpk::add_exact<T>(acc[m][1], oi);
+#pragma unroll
+                for (int j = 0; j < P; ++j)
+                {
+                    T orr, oi;
+                    fpk::cmul_exact<T>(a, b, v[2 * j], v[2 * j + 1], orr, oi);
+                    acc[m][2 * j] = fpk::add_exact<T>(acc[m][2 * j], orr);
+                    acc[m][2 * j + 1] = fpk::add_exact<T>(acc[m][2 * j + 1], oi);
+                }
             }
         }
 #pragma unroll
@@ -449,9 +462,11 @@ __global__ __launch_bounds__(B, B == kTiledBlock ? 4 : 2) void k_batch_persisten
                     __syncthreads();
                 }
                 if (op.dtype == 0)
-                    apply_coset_item<double, B>(op, wi.tile, wi.chunk, wi.begin, wi.count, smem);
+                    apply_coset_item<double, B, 1>(op, wi.tile, wi.chunk, wi.begin, wi.count, smem);
+                else if (op.nc == 2)
+                    apply_coset_item<float, B, 2>(op, wi.tile, wi.chunk, wi.begin, wi.count, smem);
                 else
-                    apply_coset_item<float, B>(op, wi.tile, wi.chunk, wi.begin, wi.count, smem);
+                    apply_coset_item<float, B, 1>(op, wi.tile, wi.chunk, wi.begin, wi.count, smem);
                 // publish this item's outputs, then count it
                 __syncthreads();
                 if (threadIdx.x == 0)
