@@ -17,8 +17,8 @@
 // the output, instead of one state read per string.
 //
 // A thread owns "slots" of the tile: one element, or (PAIRED: complex64 full
-// tiles with a power-of-two state count, S_blk >= 2) the two neighbouring
-// states of one row, moved by 16-byte loads, stores and LDS reads.  Coset bases
+// tiles with an even state count, S_blk >= 2) the two neighbouring states of one
+// row, moved by 16-byte loads, stores and LDS reads.  Coset bases
 // advance by one deposited addition per tile.  complex128 (32-bit offsets)
 // requests the next coset's output values together with its state values, in a
 // second register set; other forms request a coset's output values just before
@@ -40,8 +40,9 @@ constexpr int kThreads = 512;
 constexpr uint32_t kTileBytes = 64u * 1024u;
 
 // One group: launch over cosets [0, n_cosets) of group g, blocks striding by
-// gridDim.x; state chunk blockIdx.y.  PAIRED requires complex64, S_blk >= 2, a
-// power-of-two state count that S_blk divides, and 2^d rows >= one full tile.
+// gridDim.x; state chunk blockIdx.y.  PAIRED requires complex64, S_blk >= 2, an
+// even state count (so a slot's two states are both live or both not) and
+// 2^d rows x S_blk = one full tile.
 template <class T, bool WIDE, bool PAIRED>
 __global__ __launch_bounds__(kThreads, 4) void k_op_apply_tiled(typename fpk::VecOf<T, 1>::type const *__restrict__ states,
                                                              typename fpk::VecOf<T, 1>::type *__restrict__ out,
@@ -93,13 +94,14 @@ __global__ __launch_bounds__(kThreads, 4) void k_op_apply_tiled(typename fpk::Ve
         uint32_t const a = e >> lsbx;
         bool const live = e < tile_slots && t_live;
         uint64_t const roff = live ? (bt[a >> k] ^ uint64_t(a & lowmask)) : 0;
-        uint64_t const el = (roff << sshift) | (s0 + t0); // element index, power-of-two S
+        uint64_t const el = s_pow2 ? ((roff << sshift) | (s0 + t0)) : 0; // element index, power-of-two S
         eoff[m] = Idx(s_pow2 ? (PAIRED ? (el >> 1) : el) : roff);
     }
     // slot index of slot m in coset base cb (elements; pairs of elements when PAIRED)
     auto index_of = [&](uint64_t cb, uint32_t m) -> Idx {
         if constexpr (PAIRED)
-            return Idx(Idx(cb << (sshift - 1u)) ^ eoff[m]);
+            return s_pow2 ? Idx(Idx(cb << (sshift - 1u)) ^ eoff[m])
+                          : Idx((Idx(cb) ^ eoff[m]) * Idx(n_states >> 1) + Idx((s0 + t0) >> 1));
         else
         {
             Idx const cbs = Idx(s_pow2 ? (cb << sshift) : cb);

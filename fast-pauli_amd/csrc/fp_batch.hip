@@ -174,11 +174,12 @@ int resident_blocks(bool coset, size_t lds)
     return (e == hipSuccess && nb > 0) ? nb : 1;
 }
 
-// Coset items of complex64 entries with an even state count take two states per
-// slot (16-byte loads, stores and LDS reads).
+// Coset items of complex64 entries with an even state count (and 16-byte aligned
+// buffers) take two states per slot (16-byte loads, stores and LDS reads).
 bool coset_paired(BatchEntry const &e, fprt::TileGeom const &g)
 {
-    return e.dtype == FP_C64 && e.n_states % 2 == 0 && g.lsb >= 1;
+    bool const aligned16 = ((reinterpret_cast<uintptr_t>(e.states) | reinterpret_cast<uintptr_t>(e.out)) & 15u) == 0;
+    return e.dtype == FP_C64 && e.n_states % 2 == 0 && g.lsb >= 1 && aligned16;
 }
 
 // Coset geometry and string-group tables of an apply entry, when its state is at

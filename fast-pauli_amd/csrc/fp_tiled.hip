@@ -429,9 +429,10 @@ fp_status run_op_apply_tiled(fp_hip_ops const *ops, int dtype, void const *state
     uint64_t const blocks_x = std::max<uint64_t>(1, std::min<uint64_t>(n_cosets, (target + chunks_y - 1) / chunks_y));
     uint32_t const sshift = (n_states & (n_states - 1)) == 0 ? log2u(n_states) : 64u;
     bool const wide = (uint64_t(dim) * n_states) >> 32 != 0 || env_u32("FAST_PAULI_TILE_WIDE", 0) == 1;
-    // complex64 full tiles with a power-of-two state count that one block's
-    // states divide: two states per slot (16-byte loads, stores and LDS reads)
-    bool const paired = dtype == FP_C64 && geo.lsb >= 1 && sshift < 64u && n_states % (size_t(1) << geo.lsb) == 0 &&
+    // complex64 full tiles with an even state count: two states per slot (16-byte
+    // loads, stores and LDS reads); a slot's two states are both live or both not
+    bool const aligned16 = ((reinterpret_cast<uintptr_t>(states) | reinterpret_cast<uintptr_t>(out)) & 15u) == 0;
+    bool const paired = dtype == FP_C64 && geo.lsb >= 1 && n_states % 2 == 0 && aligned16 &&
                         geo.d + geo.lsb == log2u(fpa::kTileBytes / 8u);
     static std::once_flag attr_once;
     static hipError_t attr_err = hipSuccess;
