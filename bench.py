@@ -386,6 +386,7 @@ def main() -> int:
             "pauli_op_apply_n10": bench_op_apply_n10(args),
             "pauli_op_apply_n24": bench_op_apply_n24(args),
             "summed_expectation_n14": bench_summed_n14(args),
+            "summed_square_n14": bench_square_n14(args, failures),
             "uneven_batch_256": bench_uneven_batch(args),
             "serving": bench_serving(args),
         }
@@ -562,6 +563,58 @@ def bench_summed_n14(args) -> dict:
         res["cpu_par_ms_estimate"] = t_cpu * 1e3
         res["cpu_sample"] = "200 of 2000 strings, time x10"
         res["cpu_threads"] = O.max_threads()
+    return res
+
+
+def bench_square_n14(args, failures) -> dict:
+    """SummedPauliOp::square on the n=14 operator set (2000 strings of weight <= 3, 200
+    operators with N(0,1) weights): the 2000^2 string pairs contracted into the 2.77 M strings
+    of weight <= 6, written as a (2.77 M x 200) complex128 coefficient matrix."""
+    import ctypes
+
+    from fast_pauli import _abi, synthetic as syn
+
+    n, N, K = 14, 2000, 200
+    log("workload: SummedPauliOp.square n=14, 2000 strings, 200 operators")
+    codes = syn.random_codes_max_weight(N, n, 3, seed=args.seed)
+    W = syn.random_real_matrix(K, N, seed=args.seed)
+    h = W.T.astype(np.complex128)  # h(j, k)
+    ops = _abi.Ops(codes, matrix=h)
+    count = ctypes.c_size_t()
+    _abi.call("fp_hip_summed_square", ops.handle, _abi.FP_C128, None, ctypes.byref(count), None)
+    A = count.value
+    out = _abi.DeviceBuffer(A * K * 16)
+    ms = _time_ms(lambda: _abi.call("fp_hip_summed_square", ops.handle, _abi.FP_C128, out.ptr, ctypes.byref(count),
+                                    None), 5)
+    out_bytes = A * K * 16
+    res = {"config": "n=14, 2000 strings of weight <= 3, h (2000 x 200) ~ N(0,1) real, complex128",
+           "ms": ms, "out_strings": A, "pairs": N * N, "out_bytes": out_bytes,
+           "out_gbs": out_bytes / (ms / 1e3) / 1e9,
+           "out_frac_hbm": out_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+           "bound": "hbm (the output matrix is written once; the pair sort moves 2000^2 x 12 B per pass)"}
+    del out
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        # the whole workload through the oracle's loop (one thread), and the device result of
+        # the first 4 operators against its columns (each column's arithmetic is independent)
+        strings = syn.codes_to_strings(codes)
+        t = time.perf_counter()
+        _, exp = O.summed_square(strings, h)
+        t_cpu = time.perf_counter() - t
+        Ks = 4
+        small = _abi.Ops(codes, matrix=np.ascontiguousarray(h[:, :Ks]))
+        d = _abi.DeviceBuffer(A * Ks * 16)
+        _abi.call("fp_hip_summed_square", small.handle, _abi.FP_C128, d.ptr, ctypes.byref(count), None)
+        got = d.to_array(np.complex128, (A, Ks))
+        equal = bool(np.array_equal(got, exp[:, :Ks]))
+        res["ieee_equal_vs_oracle_4_operators"] = equal
+        if not equal:
+            failures.append("SummedPauliOp.square differs from the oracle")
+        res["cpu_ms"] = t_cpu * 1e3
+        res["cpu_sample"] = "the whole workload, one call of the oracle's loop"
+        res["cpu_threads"] = 1
+        del got, exp, d
     return res
 
 

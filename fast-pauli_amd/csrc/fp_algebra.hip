@@ -113,7 +113,7 @@ extern "C" fp_status fp_hip_summed_square(fp_hip_ops const *ops, int dtype, void
     FP_LAUNCHED();
     fpalg::RankTables const tab{W, binom, first, pow3};
     hipLaunchKernelGGL(fpalg::k_square_pairs, dim3(uint32_t((n_pairs + 255) / 256)), dim3(256), 0, st, ops->d_x,
-                       ops->d_z, N, n, tab, k_in, v_in);
+                       ops->d_z, uint32_t(N), n, tab, k_in, v_in);
     FP_LAUNCHED();
     FP_HIP(hipcub::DeviceRadixSort::SortPairs(base + o_cub, cub_bytes, k_in, k_out, v_in, v_out, int(n_pairs), 0,
                                               end_bit, st));
@@ -124,14 +124,21 @@ extern "C" fp_status fp_hip_summed_square(fp_hip_ops const *ops, int dtype, void
     if (n_work / 256 >= uint64_t(0x7fffffff))
         return fail(FP_ERR_UNSUPPORTED, "the square's coefficient matrix is too large for one launch");
     dim3 const grid(uint32_t((n_work + 255) / 256));
+    bool const small = n_work + 256 <= uint64_t(0xffffffffu);
+    typedef fpk::VecOf<double, 1>::type V2;
+    typedef fpk::VecOf<float, 1>::type F2;
+    auto const *h128 = reinterpret_cast<V2 const *>(ops->d_h128);
+    auto const *h64 = reinterpret_cast<F2 const *>(ops->d_h64);
     if (dtype == FP_C128)
-        hipLaunchKernelGGL(fpalg::k_square_contract<double>, grid, dim3(256), 0, st, v_out, starts,
-                           reinterpret_cast<fpk::VecOf<double, 1>::type const *>(ops->d_h128), N, K, A,
-                           static_cast<fpk::VecOf<double, 1>::type *>(out));
+    {
+        auto const kfn = small ? &fpalg::k_square_contract<double, true> : &fpalg::k_square_contract<double, false>;
+        hipLaunchKernelGGL(kfn, grid, dim3(256), 0, st, v_out, starts, h128, K, A, static_cast<V2 *>(out));
+    }
     else
-        hipLaunchKernelGGL(fpalg::k_square_contract<float>, grid, dim3(256), 0, st, v_out, starts,
-                           reinterpret_cast<fpk::VecOf<float, 1>::type const *>(ops->d_h64), N, K, A,
-                           static_cast<fpk::VecOf<float, 1>::type *>(out));
+    {
+        auto const kfn = small ? &fpalg::k_square_contract<float, true> : &fpalg::k_square_contract<float, false>;
+        hipLaunchKernelGGL(kfn, grid, dim3(256), 0, st, v_out, starts, h64, K, A, static_cast<F2 *>(out));
+    }
     FP_LAUNCHED();
     return FP_OK;
 }

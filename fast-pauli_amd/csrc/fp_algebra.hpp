@@ -9,13 +9,14 @@
 // term couples one pair to one output string, so the contraction has no dense
 // shared index for the matrix cores; it is a gather and segmented sum:
 //   1. k_square_pairs: pair p = i * N + j -> key a (the product's rank in that
-//      order) and value (p << 2 | phase exponent), phase_ij = i^e;
+//      order) and value (i << 33 | j << 2 | phase exponent), phase_ij = i^e
+//      (N^2 < 2^31: i and j fit 31 bits);
 //   2. a stable radix sort by key (pairs of one a stay in (i, j) order);
 //   3. k_segment_starts: first sorted position of every a (binary search);
 //   4. k_square_contract: one thread per (a, k) sums its segment in (i, j)
-//      order, (phase * h(i,k)) * h(j,k) as std::complex multiplies (no fused
-//      multiply-add), which is the reference's operation order: results are
-//      IEEE-equal to its sequential loop.
+//      order, (phase * h(i,k)) * h(j,k) as std::complex
+//      multiplies (no fused multiply-add), which is the reference's operation
+//      order: results are IEEE-equal to its sequential loop.
 //
 // Bit layout: a string is (x, z) with character c on bit n-1-c (X: x, Z: z, Y: both).
 #pragma once
@@ -73,16 +74,16 @@ __host__ __device__ inline uint64_t string_rank(uint64_t x, uint64_t z, uint32_t
 }
 
 __global__ __launch_bounds__(256) void k_square_pairs(uint64_t const *__restrict__ xs, uint64_t const *__restrict__ zs,
-                                                      uint64_t n_strings, uint32_t n, RankTables t,
+                                                      uint32_t n_strings, uint32_t n, RankTables t,
                                                       uint32_t *__restrict__ keys, uint64_t *__restrict__ vals)
 {
-    uint64_t const p = uint64_t(blockIdx.x) * 256 + threadIdx.x;
-    if (p >= n_strings * n_strings)
+    uint32_t const p = blockIdx.x * 256u + threadIdx.x;
+    if (p >= n_strings * n_strings) // < 2^31
         return;
-    uint64_t const i = p / n_strings, j = p - i * n_strings;
+    uint32_t const i = p / n_strings, j = p - i * n_strings;
     uint64_t const x1 = xs[i], z1 = zs[i], x2 = xs[j], z2 = zs[j];
     keys[p] = uint32_t(string_rank(x1 ^ x2, z1 ^ z2, n, t));
-    vals[p] = (p << 2) | product_phase(x1, z1, x2, z2);
+    vals[p] = (uint64_t(i) << 33) | (uint64_t(j) << 2) | product_phase(x1, z1, x2, z2);
 }
 
 // starts[a] = first position of key a in the sorted keys (a = 0 .. n_out inclusive).
@@ -105,25 +106,37 @@ __global__ __launch_bounds__(256) void k_segment_starts(uint32_t const *__restri
 }
 
 // out(a, k) = sum over the segment of a of (i^e * h(i,k)) * h(j,k), in sorted order.
-template <class T>
+// One thread per (a, k), k fastest; SMALL: a * n_ops + k < 2^32 (32-bit division).
+// Each output sums its own segment in sorted order.
+template <class T, bool SMALL>
 __global__ __launch_bounds__(256) void k_square_contract(uint64_t const *__restrict__ vals,
                                                          uint64_t const *__restrict__ starts,
                                                          typename fpk::VecOf<T, 1>::type const *__restrict__ h,
-                                                         uint64_t n_strings, uint64_t n_ops, uint64_t n_out,
+                                                         uint64_t n_ops, uint64_t n_out,
                                                          typename fpk::VecOf<T, 1>::type *__restrict__ out)
 {
     typedef typename fpk::VecOf<T, 1>::type V;
     uint64_t const idx = uint64_t(blockIdx.x) * 256 + threadIdx.x;
     if (idx >= n_out * n_ops)
         return;
-    uint64_t const a = idx / n_ops, k = idx - a * n_ops;
+    uint64_t a, k;
+    if constexpr (SMALL)
+    {
+        uint32_t const a32 = uint32_t(idx) / uint32_t(n_ops);
+        a = a32;
+        k = uint32_t(idx) - a32 * uint32_t(n_ops);
+    }
+    else
+    {
+        a = idx / n_ops;
+        k = idx - a * n_ops;
+    }
     T ar = T(0), ai = T(0);
     for (uint64_t s = starts[a], e = starts[a + 1]; s < e; ++s)
     {
         uint64_t const v = vals[s];
-        uint64_t const p = v >> 2;
         uint32_t const ph = uint32_t(v) & 3u;
-        uint64_t const i = p / n_strings, j = p - i * n_strings;
+        uint64_t const i = v >> 33, j = (v >> 2) & 0x7fffffffu;
         // i^ph as a complex number with exact components
         T const pr = ph == 0 ? T(1) : (ph == 2 ? T(-1) : T(0));
         T const pi = ph == 1 ? T(1) : (ph == 3 ? T(-1) : T(0));

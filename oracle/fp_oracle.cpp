@@ -697,12 +697,15 @@ void fpo_string_mul(size_t n, uint8_t const *a, uint8_t const *b, uint8_t *codes
 }
 
 // SummedPauliOp::square: returns the count A of squared strings; when the outputs are
-// not NULL, writes their codes (A x n) and coefficients (A x K complex of `dtype`).
+// not NULL and A <= capacity, writes their codes (A x n) and coefficients (A x K
+// complex of `dtype`).
 size_t fpo_summed_square(int dtype, size_t n, size_t N, size_t K, uint8_t const *codes, void const *coeffs_nk,
-                         uint8_t *codes_out, void *coeffs_out)
+                         size_t capacity, uint8_t *codes_out, void *coeffs_out)
 {
     std::vector<std::vector<uint8_t>> sq;
     auto emit = [&](auto const &c) {
+        if (sq.size() > capacity)
+            return;
         if (codes_out)
             for (size_t a = 0; a < sq.size(); ++a)
                 std::memcpy(codes_out + a * n, sq[a].data(), n);

@@ -25,6 +25,8 @@ import os
 import subprocess
 import threading
 
+import math
+
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -68,7 +70,7 @@ def lib() -> ctypes.CDLL:
             L.fpo_strings_max_weight.restype = sz
             L.fpo_string_mul.argtypes = [sz, u8p, u8p, u8p, vp]
             L.fpo_string_mul.restype = None
-            L.fpo_summed_square.argtypes = [i, sz, sz, sz, u8p, vp, u8p, vp]
+            L.fpo_summed_square.argtypes = [i, sz, sz, sz, u8p, vp, sz, u8p, vp]
             L.fpo_summed_square.restype = sz
             L.fpo_op_mul.argtypes = [sz, sz, u8p, vp, sz, u8p, vp, u8p, vp]
             L.fpo_op_mul.restype = sz
@@ -243,10 +245,14 @@ def summed_square(strings, coeffs_nk, dtype=C128) -> tuple[list[str], np.ndarray
     dt = _cdtype(dtype)
     coeffs_nk = np.ascontiguousarray(coeffs_nk, dtype=dt)
     K = coeffs_nk.shape[1]
-    A = lib().fpo_summed_square(dtype, n, N, K, _ptr(codes), _ptr(coeffs_nk), None, None)
+    # the squared strings are every string of weight <= min(n, 2 * max weight): count them
+    # here so that the (costly) loop runs once; the call returns the count it produced
+    W = min(n, 2 * int((codes != 0).sum(axis=1).max(initial=0)))
+    A = sum(3 ** w * math.comb(n, w) for w in range(W + 1))
     out_codes = np.zeros((A, n), np.uint8)
     out = np.zeros((A, K), dt)
-    lib().fpo_summed_square(dtype, n, N, K, _ptr(codes), _ptr(coeffs_nk), _ptr(out_codes), _ptr(out))
+    got = lib().fpo_summed_square(dtype, n, N, K, _ptr(codes), _ptr(coeffs_nk), A, _ptr(out_codes), _ptr(out))
+    assert got == A, (got, A)
     letters = np.array(list("IXYZ"))
     return ["".join(letters[row]) for row in out_codes], out
 
