@@ -15,6 +15,13 @@
 //                      also bit-exact).  Groups of one operator run in order:
 //                      an item of group g > 0 waits until every item of group
 //                      g - 1 has published its outputs (device-scope counter).
+//                      The outputs are read and written with agent-scope
+//                      atomic accesses, which bypass the XCDs' non-coherent L2
+//                      copies; an item's threads wait for their stores to
+//                      complete before the counter is incremented, and a waiting
+//                      item issues its output loads only after it has observed
+//                      the final count, so no L2 write-back or invalidation is
+//                      needed per item (those cost about 20 us each).
 //   expectation tile:  a range of row pairs for all strings; FP64 partials per
 //                      state.  The block that completes an operator's last tile
 //                      (device-scope counter) reduces its partials in tile
@@ -102,6 +109,36 @@ __device__ __forceinline__ double load_coherent(double const *p)
     unsigned long long const bits =
         __hip_atomic_load(reinterpret_cast<unsigned long long const *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return __longlong_as_double(static_cast<long long>(bits));
+}
+
+// Device-coherent accesses to a coset item's outputs (64-bit relaxed atomics at
+// agent scope, which bypass the XCDs' non-coherent L2 copies): one group's stores
+// reach the coherence point, where the next group's loads read them, without an
+// L2 write-back or invalidation per item.
+template <class V> __device__ __forceinline__ V load_coherent_vec(V const *p)
+{
+    static_assert(sizeof(V) % 8 == 0, "64-bit pieces");
+    constexpr int W = int(sizeof(V) / 8);
+    unsigned long long w[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i)
+        w[i] = __hip_atomic_load(reinterpret_cast<unsigned long long const *>(p) + i, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+    V v;
+    __builtin_memcpy(&v, w, sizeof(V));
+    return v;
+}
+
+template <class V> __device__ __forceinline__ void store_coherent_vec(V *p, V const &v)
+{
+    static_assert(sizeof(V) % 8 == 0, "64-bit pieces");
+    constexpr int W = int(sizeof(V) / 8);
+    unsigned long long w[W];
+    __builtin_memcpy(w, &v, sizeof(V));
+#pragma unroll
+    for (int i = 0; i < W; ++i)
+        __hip_atomic_store(reinterpret_cast<unsigned long long *>(p) + i, w[i], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <class T, int NC, int B>
@@ -259,7 +296,7 @@ __device__ void apply_coset_item(BatchOp const &op, uint32_t g, uint32_t chunk, 
         {
             acc[m] = V(0);
             if (!init_zero && slot_live(m))
-                acc[m] = out[index_of(cbl, m)];
+                acc[m] = load_coherent_vec(&out[index_of(cbl, m)]);
         }
         __syncthreads(); // the previous tile is no longer read
 #pragma unroll
@@ -314,7 +351,7 @@ __device__ void apply_coset_item(BatchOp const &op, uint32_t g, uint32_t chunk, 
 #pragma unroll
         for (uint32_t m = 0; m < EPT; ++m)
             if (slot_live(m))
-                __builtin_nontemporal_store(acc[m], &out[index_of(cbl, m)]);
+                store_coherent_vec(&out[index_of(cbl, m)], acc[m]);
         cb = cbn;
     }
 }
@@ -449,7 +486,7 @@ __global__ __launch_bounds__(B, B == kTiledBlock ? 4 : 2) void k_batch_persisten
                     {
                         uint32_t const *c = done + op.dep_base + wi.tile - 1;
                         uint32_t spins = 0;
-                        while (__hip_atomic_load(c, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < op.items_per_group)
+                        while (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < op.items_per_group)
                         {
                             if (++spins > kSpinLimit)
                             {
@@ -467,13 +504,12 @@ __global__ __launch_bounds__(B, B == kTiledBlock ? 4 : 2) void k_batch_persisten
                     apply_coset_item<float, B, 2>(op, wi.tile, wi.chunk, wi.begin, wi.count, smem);
                 else
                     apply_coset_item<float, B, 1>(op, wi.tile, wi.chunk, wi.begin, wi.count, smem);
-                // publish this item's outputs, then count it
+                // count this item once every thread's output stores have completed (they
+                // are device-coherent, so no cache write-back is needed to publish them)
+                __builtin_amdgcn_s_waitcnt(0);
                 __syncthreads();
                 if (threadIdx.x == 0)
-                {
-                    __threadfence();
-                    atomicAdd(done + op.dep_base + wi.tile, 1u);
-                }
+                    __hip_atomic_fetch_add(done + op.dep_base + wi.tile, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
         else if (op.kind == kApply)
