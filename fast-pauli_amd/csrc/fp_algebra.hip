@@ -5,6 +5,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "fast_pauli_hip.h"
@@ -123,8 +124,19 @@ extern "C" fp_status fp_hip_summed_square(fp_hip_ops const *ops, int dtype, void
     uint64_t const n_work = A * K;
     if (n_work / 256 >= uint64_t(0x7fffffff))
         return fail(FP_ERR_UNSUPPORTED, "the square's coefficient matrix is too large for one launch");
-    dim3 const grid(uint32_t((n_work + 255) / 256));
-    bool const small = n_work + 256 <= uint64_t(0xffffffffu);
+    // a resident grid (FAST_PAULI_SQUARE_BLOCKS_PER_CU blocks per CU, default 32: the
+    // contraction is bound by the latency of its dependent loads, so more waves help)
+    static int const cus = [] {
+        int dev = 0, n = 256;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            n = prop.multiProcessorCount;
+        return n;
+    }();
+    char const *bpc_env = std::getenv("FAST_PAULI_SQUARE_BLOCKS_PER_CU");
+    uint64_t const bpc = bpc_env && std::atoi(bpc_env) > 0 ? uint64_t(std::atoi(bpc_env)) : 32;
+    dim3 const grid(uint32_t(std::min<uint64_t>((n_work + 255) / 256, uint64_t(cus) * bpc)));
+    bool const small = n_work + uint64_t(grid.x) * 256 <= uint64_t(0xffffffffu);
     typedef fpk::VecOf<double, 1>::type V2;
     typedef fpk::VecOf<float, 1>::type F2;
     auto const *h128 = reinterpret_cast<V2 const *>(ops->d_h128);

@@ -106,8 +106,10 @@ __global__ __launch_bounds__(256) void k_segment_starts(uint32_t const *__restri
 }
 
 // out(a, k) = sum over the segment of a of (i^e * h(i,k)) * h(j,k), in sorted order.
-// One thread per (a, k), k fastest; SMALL: a * n_ops + k < 2^32 (32-bit division).
-// Each output sums its own segment in sorted order.
+// Threads stride over the (a, k) outputs, k fastest (a resident grid: the outputs
+// are many and short, so the waves stay resident instead of being launched one
+// per 64 outputs); SMALL: every index < 2^32 (32-bit division).  Each output sums
+// its own segment in sorted order.
 template <class T, bool SMALL>
 __global__ __launch_bounds__(256) void k_square_contract(uint64_t const *__restrict__ vals,
                                                          uint64_t const *__restrict__ starts,
@@ -116,41 +118,43 @@ __global__ __launch_bounds__(256) void k_square_contract(uint64_t const *__restr
                                                          typename fpk::VecOf<T, 1>::type *__restrict__ out)
 {
     typedef typename fpk::VecOf<T, 1>::type V;
-    uint64_t const idx = uint64_t(blockIdx.x) * 256 + threadIdx.x;
-    if (idx >= n_out * n_ops)
-        return;
-    uint64_t a, k;
-    if constexpr (SMALL)
+    uint64_t const n_work = n_out * n_ops;
+    uint64_t const stride = uint64_t(gridDim.x) * 256;
+    for (uint64_t idx = uint64_t(blockIdx.x) * 256 + threadIdx.x; idx < n_work; idx += stride)
     {
-        uint32_t const a32 = uint32_t(idx) / uint32_t(n_ops);
-        a = a32;
-        k = uint32_t(idx) - a32 * uint32_t(n_ops);
+        uint64_t a, k;
+        if constexpr (SMALL)
+        {
+            uint32_t const a32 = uint32_t(idx) / uint32_t(n_ops);
+            a = a32;
+            k = uint32_t(idx) - a32 * uint32_t(n_ops);
+        }
+        else
+        {
+            a = idx / n_ops;
+            k = idx - a * n_ops;
+        }
+        T ar = T(0), ai = T(0);
+        for (uint64_t s = starts[a], e = starts[a + 1]; s < e; ++s)
+        {
+            uint64_t const v = vals[s];
+            uint32_t const ph = uint32_t(v) & 3u;
+            uint64_t const i = v >> 33, j = (v >> 2) & 0x7fffffffu;
+            // i^ph as a complex number with exact components
+            T const pr = ph == 0 ? T(1) : (ph == 2 ? T(-1) : T(0));
+            T const pi = ph == 1 ? T(1) : (ph == 3 ? T(-1) : T(0));
+            V const hi = h[i * n_ops + k], hj = h[j * n_ops + k];
+            T tr, ti, ur, ui;
+            fpk::cmul_exact<T>(pr, pi, hi[0], hi[1], tr, ti);
+            fpk::cmul_exact<T>(tr, ti, hj[0], hj[1], ur, ui);
+            ar = fpk::add_exact<T>(ar, ur);
+            ai = fpk::add_exact<T>(ai, ui);
+        }
+        V r;
+        r[0] = ar;
+        r[1] = ai;
+        out[idx] = r;
     }
-    else
-    {
-        a = idx / n_ops;
-        k = idx - a * n_ops;
-    }
-    T ar = T(0), ai = T(0);
-    for (uint64_t s = starts[a], e = starts[a + 1]; s < e; ++s)
-    {
-        uint64_t const v = vals[s];
-        uint32_t const ph = uint32_t(v) & 3u;
-        uint64_t const i = v >> 33, j = (v >> 2) & 0x7fffffffu;
-        // i^ph as a complex number with exact components
-        T const pr = ph == 0 ? T(1) : (ph == 2 ? T(-1) : T(0));
-        T const pi = ph == 1 ? T(1) : (ph == 3 ? T(-1) : T(0));
-        V const hi = h[i * n_ops + k], hj = h[j * n_ops + k];
-        T tr, ti, ur, ui;
-        fpk::cmul_exact<T>(pr, pi, hi[0], hi[1], tr, ti);
-        fpk::cmul_exact<T>(tr, ti, hj[0], hj[1], ur, ui);
-        ar = fpk::add_exact<T>(ar, ur);
-        ai = fpk::add_exact<T>(ai, ui);
-    }
-    V r;
-    r[0] = ar;
-    r[1] = ai;
-    out[idx] = r;
 }
 
 } // namespace fpalg
