@@ -387,6 +387,7 @@ def main() -> int:
             "pauli_op_apply_n24": bench_op_apply_n24(args),
             "summed_expectation_n14": bench_summed_n14(args),
             "summed_square_n14": bench_square_n14(args, failures),
+            "summed_apply_n14": bench_summed_apply_n14(args, failures),
             "uneven_batch_256": bench_uneven_batch(args),
             "serving": bench_serving(args),
         }
@@ -562,6 +563,74 @@ def bench_summed_n14(args) -> dict:
         t_cpu = (time.perf_counter() - t) * (N / 200)
         res["cpu_par_ms_estimate"] = t_cpu * 1e3
         res["cpu_sample"] = "200 of 2000 strings, time x10"
+        res["cpu_threads"] = O.max_threads()
+    return res
+
+
+def bench_summed_apply_n14(args, failures) -> dict:
+    """SummedPauliOp::apply and apply_weighted on the n=14 operator set (2000 strings of
+    weight <= 3, 200 operators) and 1000 states, complex128, streaming kernels: every
+    output element sums its 2000 strings' terms in order (one partner read per string)."""
+    from fast_pauli import _abi, synthetic as syn
+
+    n, N, K, S = 14, 2000, 200, 1000
+    dim = 1 << n
+    log("workload: SummedPauliOp.apply / apply_weighted n=14, 2000 strings, 200 operators, 1000 states")
+    codes = syn.random_codes_max_weight(N, n, 3, seed=args.seed)
+    W = syn.random_real_matrix(K, N, seed=args.seed)
+    h = W.T.astype(np.complex128)
+    ops = _abi.Ops(codes, matrix=h)
+    x = syn.random_real_matrix(K, S, seed=args.seed + 1)
+    d_in = _abi.DeviceBuffer(dim * S * 16)
+    _abi.call("fp_hip_fill_states", _abi.FP_C128, d_in.ptr, dim, S, syn.gpu_seed(args.seed), 1, None)
+    d_x = _abi.DeviceBuffer.from_array(np.ascontiguousarray(x, np.float64))
+    d_apply = _abi.DeviceBuffer(dim * S * 16)
+    d_weighted = _abi.DeviceBuffer(dim * S * 16)
+    ms_apply = _time_ms(lambda: _abi.call("fp_hip_summed_apply", ops.handle, _abi.FP_C128, d_in.ptr, d_apply.ptr, dim,
+                                          S, 0, None), 3)
+    ms_w = _time_ms(lambda: _abi.call("fp_hip_summed_apply_weighted", ops.handle, _abi.FP_C128, d_in.ptr, d_x.ptr,
+                                      d_weighted.ptr, dim, S, 0, None), 3)
+    state_bytes = dim * S * 16
+    model = (N + 1) * state_bytes  # one partner read per string, one output write
+    res = {"config": "n=14, 2000 strings of weight <= 3, h (2000 x 200) ~ N(0,1) real, x (200 x 1000) ~ N(0,1), "
+                     "1000 normalised states, complex128",
+           "apply_ms": ms_apply, "apply_weighted_ms": ms_w,
+           "model_bytes": model, "bytes_rule": "(strings + 1) x 2^n x S x 16 B",
+           "apply_gbs": model / (ms_apply / 1e3) / 1e9, "apply_frac_hbm": model / (ms_apply / 1e3) / 1e9 / HBM_PEAK_GBS,
+           "apply_weighted_gbs": model / (ms_w / 1e3) / 1e9,
+           "strings_x_states_per_s": N * S / (ms_apply / 1e3)}
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        # bounded samples (state columns are independent): the reference's sequential loop
+        # on 4 states is the bit-exact check; its parallel branch (per-thread partial sums,
+        # another summation order) on 16 states is the timed baseline
+        Sp, Ss = 4, 16
+        psi = d_in.to_array(np.complex128, (dim, S))[:, :Ss].copy()
+        xs = np.ascontiguousarray(x[:, :Ss])
+        strings = syn.codes_to_strings(codes)
+        got_a = d_apply.to_array(np.complex128, (dim, S))[:, :Ss]
+        got_w = d_weighted.to_array(np.complex128, (dim, S))[:, :Ss]
+        seq_a = O.summed_apply(strings, h, psi[:, :Sp].copy(), np.zeros((dim, Sp), np.complex128))
+        seq_w = O.summed_apply_weighted(strings, h, psi[:, :Sp].copy(), np.ascontiguousarray(xs[:, :Sp]),
+                                        np.zeros((dim, Sp), np.complex128))
+        res["apply_ieee_equal_vs_oracle_seq_4_states"] = bool(np.array_equal(got_a[:, :Sp], seq_a))
+        res["apply_weighted_ieee_equal_vs_oracle_seq_4_states"] = bool(np.array_equal(got_w[:, :Sp], seq_w))
+        if not res["apply_ieee_equal_vs_oracle_seq_4_states"]:
+            failures.append("SummedPauliOp.apply differs from the oracle's sequential loop")
+        if not res["apply_weighted_ieee_equal_vs_oracle_seq_4_states"]:
+            failures.append("SummedPauliOp.apply_weighted differs from the oracle's sequential loop")
+        t = time.perf_counter()
+        par_a = O.summed_apply(strings, h, psi, np.zeros_like(psi), par=True)
+        t_a = time.perf_counter() - t
+        t = time.perf_counter()
+        par_w = O.summed_apply_weighted(strings, h, psi, xs, np.zeros_like(psi), par=True)
+        t_w = time.perf_counter() - t
+        res["apply_rel_diff_vs_oracle_par"] = float(np.linalg.norm(got_a - par_a) / np.linalg.norm(par_a))
+        res["apply_weighted_rel_diff_vs_oracle_par"] = float(np.linalg.norm(got_w - par_w) / np.linalg.norm(par_w))
+        res["cpu_apply_ms_estimate"] = t_a * 1e3 * S / Ss
+        res["cpu_apply_weighted_ms_estimate"] = t_w * 1e3 * S / Ss
+        res["cpu_sample"] = f"{Ss} of {S} states, time x{S // Ss} (states run independently)"
         res["cpu_threads"] = O.max_threads()
     return res
 
