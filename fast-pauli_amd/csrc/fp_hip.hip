@@ -17,6 +17,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "fp_dense.hpp"
@@ -1514,56 +1515,62 @@ fp_status fp_hip_summed_apply_weighted(fp_hip_ops const *ops, int dtype, void co
     FP_LAUNCHED();
     VecGeom const g = vec_geom(dtype, dim, n_states, states, out);
     bool const acc = accumulate != 0;
-    for (uint64_t v0 = 0; v0 < g.n_vec; v0 += kMaxLaunchBlocks * fpk::kBlock)
-    {
-        uint32_t const blocks =
-            uint32_t(std::min<uint64_t>((g.n_vec - v0 + fpk::kBlock - 1) / fpk::kBlock, kMaxLaunchBlocks));
-        auto launch = [&](auto kernel, auto const *in_p, auto *out_p, auto const *w_p) {
-            hipLaunchKernelGGL(kernel, dim3(blocks), dim3(fpk::kBlock), 0, s, in_p, out_p, v0, g.n_vec, g.shift,
-                               g.vpr, g.inv_vpr, ops->d_x, ops->d_z, ops->d_ny, w_p, uint32_t(N), uint64_t(S));
-        };
-        if (dtype == FP_C128)
+    // four rows per thread (one for a 1-qubit operator, whose 2 rows do not split)
+    auto run = [&](auto rows) -> fp_status {
+        constexpr int R = decltype(rows)::value;
+        uint64_t const n_slot = g.n_vec / R;
+        for (uint64_t v0 = 0; v0 < n_slot; v0 += kMaxLaunchBlocks * fpk::kBlock)
         {
-            using V = fpk::VecOf<double, 1>::type;
-            auto const *ip = static_cast<V const *>(states);
-            auto *op = static_cast<V *>(out);
-            auto const *wp = static_cast<double const *>(scratch);
-            if (g.pow2)
-                acc ? launch(fpk::k_op_apply_weighted<double, 1, true, true>, ip, op, wp)
-                    : launch(fpk::k_op_apply_weighted<double, 1, true, false>, ip, op, wp);
+            uint32_t const blocks =
+                uint32_t(std::min<uint64_t>((n_slot - v0 + fpk::kBlock - 1) / fpk::kBlock, kMaxLaunchBlocks));
+            auto launch = [&](auto kernel, auto const *in_p, auto *out_p, auto const *w_p) {
+                hipLaunchKernelGGL(kernel, dim3(blocks), dim3(fpk::kBlock), 0, s, in_p, out_p, v0, g.n_vec, g.shift,
+                                   g.vpr, g.inv_vpr, ops->d_x, ops->d_z, ops->d_ny, w_p, uint32_t(N), uint64_t(S));
+            };
+            if (dtype == FP_C128)
+            {
+                using V = fpk::VecOf<double, 1>::type;
+                auto const *ip = static_cast<V const *>(states);
+                auto *op = static_cast<V *>(out);
+                auto const *wp = static_cast<double const *>(scratch);
+                if (g.pow2)
+                    acc ? launch(fpk::k_op_apply_weighted<double, 1, true, true, R>, ip, op, wp)
+                        : launch(fpk::k_op_apply_weighted<double, 1, true, false, R>, ip, op, wp);
+                else
+                    acc ? launch(fpk::k_op_apply_weighted<double, 1, false, true, R>, ip, op, wp)
+                        : launch(fpk::k_op_apply_weighted<double, 1, false, false, R>, ip, op, wp);
+            }
+            else if (g.nc == 2)
+            {
+                using V = fpk::VecOf<float, 2>::type;
+                auto const *ip = static_cast<V const *>(states);
+                auto *op = static_cast<V *>(out);
+                auto const *wp = static_cast<float const *>(scratch);
+                if (g.pow2)
+                    acc ? launch(fpk::k_op_apply_weighted<float, 2, true, true, R>, ip, op, wp)
+                        : launch(fpk::k_op_apply_weighted<float, 2, true, false, R>, ip, op, wp);
+                else
+                    acc ? launch(fpk::k_op_apply_weighted<float, 2, false, true, R>, ip, op, wp)
+                        : launch(fpk::k_op_apply_weighted<float, 2, false, false, R>, ip, op, wp);
+            }
             else
-                acc ? launch(fpk::k_op_apply_weighted<double, 1, false, true>, ip, op, wp)
-                    : launch(fpk::k_op_apply_weighted<double, 1, false, false>, ip, op, wp);
+            {
+                using V = fpk::VecOf<float, 1>::type;
+                auto const *ip = static_cast<V const *>(states);
+                auto *op = static_cast<V *>(out);
+                auto const *wp = static_cast<float const *>(scratch);
+                if (g.pow2)
+                    acc ? launch(fpk::k_op_apply_weighted<float, 1, true, true, R>, ip, op, wp)
+                        : launch(fpk::k_op_apply_weighted<float, 1, true, false, R>, ip, op, wp);
+                else
+                    acc ? launch(fpk::k_op_apply_weighted<float, 1, false, true, R>, ip, op, wp)
+                        : launch(fpk::k_op_apply_weighted<float, 1, false, false, R>, ip, op, wp);
+            }
+            FP_LAUNCHED();
         }
-        else if (g.nc == 2)
-        {
-            using V = fpk::VecOf<float, 2>::type;
-            auto const *ip = static_cast<V const *>(states);
-            auto *op = static_cast<V *>(out);
-            auto const *wp = static_cast<float const *>(scratch);
-            if (g.pow2)
-                acc ? launch(fpk::k_op_apply_weighted<float, 2, true, true>, ip, op, wp)
-                    : launch(fpk::k_op_apply_weighted<float, 2, true, false>, ip, op, wp);
-            else
-                acc ? launch(fpk::k_op_apply_weighted<float, 2, false, true>, ip, op, wp)
-                    : launch(fpk::k_op_apply_weighted<float, 2, false, false>, ip, op, wp);
-        }
-        else
-        {
-            using V = fpk::VecOf<float, 1>::type;
-            auto const *ip = static_cast<V const *>(states);
-            auto *op = static_cast<V *>(out);
-            auto const *wp = static_cast<float const *>(scratch);
-            if (g.pow2)
-                acc ? launch(fpk::k_op_apply_weighted<float, 1, true, true>, ip, op, wp)
-                    : launch(fpk::k_op_apply_weighted<float, 1, true, false>, ip, op, wp);
-            else
-                acc ? launch(fpk::k_op_apply_weighted<float, 1, false, true>, ip, op, wp)
-                    : launch(fpk::k_op_apply_weighted<float, 1, false, false>, ip, op, wp);
-        }
-        FP_LAUNCHED();
-    }
-    return FP_OK;
+        return FP_OK;
+    };
+    return dim % 4 == 0 ? run(std::integral_constant<int, 4>{}) : run(std::integral_constant<int, 1>{});
 }
 
 fp_status fp_hip_gemm(int contraction, size_t m, size_t n, size_t k, void const *a, void const *b, void *c,

@@ -686,7 +686,11 @@ __global__ __launch_bounds__(kBlock) void k_weighted_coeffs(T const *__restrict_
     w[2 * i + 1] = wi;
 }
 
-template <class T, int NC, bool POW2, bool ACC>
+// R rows per thread (rows row0 + u * dim / R, one column): the per-(string, state)
+// coefficient w(s, t) is read once for the R output elements that use it, and the
+// R partner reads per string are independent.  Each element sums its strings in
+// order, as the reference's sequential loop does.
+template <class T, int NC, bool POW2, bool ACC, int R>
 __global__ __launch_bounds__(kBlock) void k_op_apply_weighted(typename VecOf<T, NC>::type const *__restrict__ in,
                                                               typename VecOf<T, NC>::type *__restrict__ out,
                                                               uint64_t v_begin, uint64_t n_vec, uint32_t vpr_shift,
@@ -698,61 +702,86 @@ __global__ __launch_bounds__(kBlock) void k_op_apply_weighted(typename VecOf<T, 
                                                               uint64_t n_states)
 {
     typedef typename VecOf<T, NC>::type V;
+    // n_vec / R thread slots; slot v covers vectors v + u * (n_vec / R)
+    uint64_t const n_slot = n_vec / R;
     uint64_t const v = v_begin + uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (v >= n_vec)
+    if (v >= n_slot)
         return;
-    uint64_t row, col;
+    uint64_t row0, col;
     if constexpr (POW2)
     {
-        row = v >> vpr_shift;
+        row0 = v >> vpr_shift;
         col = v & ((uint64_t(1) << vpr_shift) - 1);
     }
     else
     {
-        row = row_of(v, vpr, inv_vpr);
-        col = v - row * vpr;
+        row0 = row_of(v, vpr, inv_vpr);
+        col = v - row0 * vpr;
     }
-    V acc = ACC ? out[v] : V(0);
+    uint64_t const row_step = n_slot / vpr; // dim / R
+    V acc[R];
+#pragma unroll
+    for (int u = 0; u < R; ++u)
+        acc[u] = ACC ? out[v + uint64_t(u) * n_slot] : V(0);
     for (uint32_t s = 0; s < n_strings; ++s)
     {
         uint64_t const x = xs[s];
-        uint64_t const sv = POW2 ? (((row ^ x) << vpr_shift) | col) : ((row ^ x) * vpr + col);
-        V const p = in[sv];
-        uint32_t const sg = parity64(row & zs[s]);
+        uint64_t const z = zs[s];
         uint32_t const ny = nys[s] & 3u;
-        // m = (-i)^{ny} * (-1)^{sg}: (re, im) of m * (pr + i pi) without rounding
+        T wr[NC], wi[NC];
 #pragma unroll
         for (int c = 0; c < NC; ++c)
         {
-            T pr = p[2 * c], pi = p[2 * c + 1];
-            T ar, ai;
-            switch (ny)
-            {
-            case 0:
-                ar = pr, ai = pi;
-                break;
-            case 1:
-                ar = pi, ai = -pr;
-                break;
-            case 2:
-                ar = -pr, ai = -pi;
-                break;
-            default:
-                ar = -pi, ai = pr;
-                break;
-            }
-            if (sg)
-                ar = -ar, ai = -ai;
             uint64_t const t = col * NC + c;
-            T const wr = w[2 * (uint64_t(s) * n_states + t)];
-            T const wi = w[2 * (uint64_t(s) * n_states + t) + 1];
-            T orr, oi;
-            cmul_exact<T>(ar, ai, wr, wi, orr, oi);
-            acc[2 * c] = add_exact<T>(acc[2 * c], orr);
-            acc[2 * c + 1] = add_exact<T>(acc[2 * c + 1], oi);
+            wr[c] = w[2 * (uint64_t(s) * n_states + t)];
+            wi[c] = w[2 * (uint64_t(s) * n_states + t) + 1];
+        }
+        V p[R];
+#pragma unroll
+        for (int u = 0; u < R; ++u)
+        {
+            uint64_t const row = row0 + uint64_t(u) * row_step;
+            uint64_t const sv = POW2 ? (((row ^ x) << vpr_shift) | col) : ((row ^ x) * vpr + col);
+            p[u] = in[sv];
+        }
+#pragma unroll
+        for (int u = 0; u < R; ++u)
+        {
+            uint64_t const row = row0 + uint64_t(u) * row_step;
+            uint32_t const sg = parity64(row & z);
+            // m = (-i)^{ny} * (-1)^{sg}: (re, im) of m * (pr + i pi) without rounding
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+            {
+                T pr = p[u][2 * c], pi = p[u][2 * c + 1];
+                T ar, ai;
+                switch (ny)
+                {
+                case 0:
+                    ar = pr, ai = pi;
+                    break;
+                case 1:
+                    ar = pi, ai = -pr;
+                    break;
+                case 2:
+                    ar = -pr, ai = -pi;
+                    break;
+                default:
+                    ar = -pi, ai = pr;
+                    break;
+                }
+                if (sg)
+                    ar = -ar, ai = -ai;
+                T orr, oi;
+                cmul_exact<T>(ar, ai, wr[c], wi[c], orr, oi);
+                acc[u][2 * c] = add_exact<T>(acc[u][2 * c], orr);
+                acc[u][2 * c + 1] = add_exact<T>(acc[u][2 * c + 1], oi);
+            }
         }
     }
-    out[v] = acc;
+#pragma unroll
+    for (int u = 0; u < R; ++u)
+        out[v + uint64_t(u) * n_slot] = acc[u];
 }
 
 // Plain streaming copy (16-byte vectors, non-temporal): the reference point for
