@@ -591,12 +591,16 @@ def bench_summed_apply_n14(args, failures) -> dict:
     ms_w = _time_ms(lambda: _abi.call("fp_hip_summed_apply_weighted", ops.handle, _abi.FP_C128, d_in.ptr, d_x.ptr,
                                       d_weighted.ptr, dim, S, 0, None), 3)
     state_bytes = dim * S * 16
-    model = (N + 1) * state_bytes  # one partner read per string, one output write
+    model = (N + 1) * state_bytes  # streaming: one partner read per string, one output write
+    plan = _abi.summed_apply_plan(ops, _abi.FP_C128, dim, S)
     res = {"config": "n=14, 2000 strings of weight <= 3, h (2000 x 200) ~ N(0,1) real, x (200 x 1000) ~ N(0,1), "
                      "1000 normalised states, complex128",
            "apply_ms": ms_apply, "apply_weighted_ms": ms_w,
-           "model_bytes": model, "bytes_rule": "(strings + 1) x 2^n x S x 16 B",
-           "apply_gbs": model / (ms_apply / 1e3) / 1e9, "apply_frac_hbm": model / (ms_apply / 1e3) / 1e9 / HBM_PEAK_GBS,
+           "apply_kernel": "coset-tiled" if plan["kernel"] == _abi.FP_EV_TILED else "streaming",
+           "apply_groups": plan["n_groups"], "apply_plan_bytes": plan["hbm_bytes"],
+           "apply_plan_gbs": plan["hbm_bytes"] / (ms_apply / 1e3) / 1e9,
+           "apply_plan_frac_hbm": plan["hbm_bytes"] / (ms_apply / 1e3) / 1e9 / HBM_PEAK_GBS,
+           "streaming_model_bytes": model, "streaming_bytes_rule": "(strings + 1) x 2^n x S x 16 B",
            "apply_weighted_gbs": model / (ms_w / 1e3) / 1e9,
            "strings_x_states_per_s": N * S / (ms_apply / 1e3)}
     if not args.no_cpu:

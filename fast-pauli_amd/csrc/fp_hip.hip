@@ -1195,6 +1195,7 @@ fp_status fp_hip_ops_set_matrix(fp_hip_ops *ops, size_t n_operators, double cons
         return fail(FP_ERR_INVALID_ARGUMENT, "empty coefficient matrix");
     (void)hipDeviceSynchronize();
     ++ops->generation;
+    fprt::invalidate_tiles(ops); // the tiled SummedPauliOp::apply tables hold the row sums
     size_t const N = ops->n_strings, K = n_operators;
     ops->n_operators = K;
     ops->matrix.assign(coeffs_nk, coeffs_nk + 2 * N * K);
@@ -1248,6 +1249,8 @@ fp_status fp_hip_ops_set_matrix(fp_hip_ops *ops, size_t n_operators, double cons
             *p = nullptr;
         }
     }
+    ops->rowsum128 = rowsum_cf;
+    ops->rowsum64 = rowsum_cf32;
     if ((st = upload(&ops->d_wr64, wr)) != FP_OK || (st = upload(&ops->d_wi64, wi)) != FP_OK ||
         (st = upload(&ops->d_wr32, wr32)) != FP_OK || (st = upload(&ops->d_wi32, wi32)) != FP_OK ||
         (st = upload(&ops->d_rowsum128, rowsum_cf)) != FP_OK || (st = upload(&ops->d_rowsum64, rowsum_cf32)) != FP_OK ||
@@ -1469,6 +1472,12 @@ fp_status fp_hip_summed_apply(fp_hip_ops const *ops, int dtype, void const *stat
     if ((st = check_no_alias(dtype, states, out, dim, n_states)) != FP_OK)
         return st;
     hipStream_t const s = static_cast<hipStream_t>(stream);
+    // large states whose strings share coset tiles: the tiled kernel (same operations, same order)
+    bool use_tiled = false;
+    if ((st = fprt::choose_op_apply_tiled(ops, dtype, n_states, &use_tiled, true)) != FP_OK)
+        return st;
+    if (use_tiled)
+        return fprt::run_op_apply_tiled(ops, dtype, states, out, dim, n_states, accumulate, s, true);
     VecGeom const g = vec_geom(dtype, dim, n_states, states, out);
     uint32_t const N = uint32_t(ops->n_strings);
     if (dtype == FP_C128)

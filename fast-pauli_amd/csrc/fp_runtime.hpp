@@ -129,9 +129,11 @@ fp_status run_ev_coset(fp_hip_ops const *ops, int dtype, void const *states, voi
                        void *scratch, hipStream_t st);
 void release_tiles(fp_hip_ops *ops);
 fp_status choose_op_expect_tiled(fp_hip_ops const *ops, int dtype, size_t n_states, bool *use_tiled);
-fp_status choose_op_apply_tiled(fp_hip_ops const *ops, int dtype, size_t n_states, bool *use_tiled);
+// `summed`: SummedPauliOp::apply (the row-summed coefficients) instead of PauliOp::apply
+fp_status choose_op_apply_tiled(fp_hip_ops const *ops, int dtype, size_t n_states, bool *use_tiled,
+                                bool summed = false);
 fp_status run_op_apply_tiled(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim,
-                             size_t n_states, int accumulate, hipStream_t st);
+                             size_t n_states, int accumulate, hipStream_t st, bool summed = false);
 // With `ws`, the string-group tables and the partials go into the workspace (no
 // allocation; fails with FP_ERR_INVALID_ARGUMENT when they cannot fit), else the
 // tables' device copy and the stream's scratch.
@@ -203,6 +205,8 @@ struct fp_hip_ops
     uint32_t n_ev_wht = 0; // the first n_ev_wht chunks are transform classes
     double *d_rowsum128 = nullptr; // SummedPauliOp::apply folded coefficients (complex128)
     float *d_rowsum64 = nullptr;
+    std::vector<double> rowsum128; // host copies of the same (the tiled path's tables)
+    std::vector<float> rowsum64;
     // contraction operands: W^T as (n_operators x n_strings) real and imaginary parts
     double *d_wr64 = nullptr, *d_wi64 = nullptr;
     float *d_wr32 = nullptr, *d_wi32 = nullptr;

@@ -53,8 +53,17 @@ fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, 
             *out = &t;
             return FP_OK;
         }
-    StringTable const st{ops->n_qubits, ops->n_strings, ops->x.data(),      ops->z.data(),
-                         ops->ny.data(),  ops->coeffs.data(), ops->cf128.data(), ops->cf64.data()};
+    bool const rows = kind == kTilesSummedApply;
+    if (rows && (ops->rowsum128.size() != 2 * ops->n_strings || ops->rowsum64.size() != 2 * ops->n_strings))
+        return fail(FP_ERR_INVALID_ARGUMENT, "SummedPauliOp has no coefficient matrix");
+    StringTable const st{ops->n_qubits,
+                         ops->n_strings,
+                         ops->x.data(),
+                         ops->z.data(),
+                         ops->ny.data(),
+                         ops->coeffs.data(),
+                         rows ? ops->rowsum128.data() : ops->cf128.data(),
+                         rows ? ops->rowsum64.data() : ops->cf64.data()};
     TileImage img;
     fp_status const bs = build_tile_image(st, kind, k, dh, wht_min, img);
     if (bs != FP_OK)
@@ -387,7 +396,7 @@ fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *stat
 // "tiled") or, by default, when the state is large (64 MiB and up: beyond the
 // caches that serve the streaming kernel's repeated row reads) and the groups'
 // traffic (about three state passes each) is below the strings' (one each).
-fp_status choose_op_apply_tiled(fp_hip_ops const *ops, int dtype, size_t n_states, bool *use_tiled)
+fp_status choose_op_apply_tiled(fp_hip_ops const *ops, int dtype, size_t n_states, bool *use_tiled, bool summed)
 {
     *use_tiled = false;
     char const *v = std::getenv("FAST_PAULI_APPLY_KERNEL");
@@ -399,7 +408,7 @@ fp_status choose_op_apply_tiled(fp_hip_ops const *ops, int dtype, size_t n_state
         return FP_OK;
     TileGeom const geo = tile_geom(dtype, ops->n_qubits, n_states, false);
     TileTables *tt = nullptr;
-    fp_status const st = tile_tables(ops, kTilesOpApply, geo.k, geo.dh, &tt);
+    fp_status const st = tile_tables(ops, summed ? kTilesSummedApply : kTilesOpApply, geo.k, geo.dh, &tt);
     if (st != FP_OK)
         return st;
     *use_tiled = forced || 3.0 * double(tt->n_groups) <= 0.8 * double(ops->n_strings);
@@ -407,11 +416,11 @@ fp_status choose_op_apply_tiled(fp_hip_ops const *ops, int dtype, size_t n_state
 }
 
 fp_status run_op_apply_tiled(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim,
-                             size_t n_states, int accumulate, hipStream_t st)
+                             size_t n_states, int accumulate, hipStream_t st, bool summed)
 {
     TileGeom const geo = tile_geom(dtype, ops->n_qubits, n_states, false);
     TileTables *tt = nullptr;
-    fp_status s = tile_tables(ops, kTilesOpApply, geo.k, geo.dh, &tt);
+    fp_status s = tile_tables(ops, summed ? kTilesSummedApply : kTilesOpApply, geo.k, geo.dh, &tt);
     if (s != FP_OK)
         return s;
     if ((s = check_tile_fit(tt, geo, dtype, fpa::kTileBytes)) != FP_OK)
@@ -637,8 +646,11 @@ extern "C" fp_status fp_hip_op_expectation_plan(fp_hip_ops const *ops, int dtype
     return FP_OK;
 }
 
-extern "C" fp_status fp_hip_op_apply_plan(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_states, int accumulate,
-                                          fp_hip_ev_plan *plan)
+namespace
+{
+
+fp_status apply_plan(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_states, int accumulate, bool summed,
+                     fp_hip_ev_plan *plan)
 {
     if (!ops || !plan)
         return fail(FP_ERR_INVALID_ARGUMENT, "NULL ops or plan");
@@ -647,8 +659,10 @@ extern "C" fp_status fp_hip_op_apply_plan(fp_hip_ops const *ops, int dtype, size
     if (n_states == 0 || dim != (size_t(1) << ops->n_qubits))
         return fail(FP_ERR_INVALID_ARGUMENT, "shape (%zu, %zu) does not match %zu qubits", dim, n_states,
                     ops->n_qubits);
+    if (summed && ops->n_operators == 0)
+        return fail(FP_ERR_INVALID_ARGUMENT, "SummedPauliOp has no coefficient matrix");
     bool tiled = false;
-    fp_status st = fprt::choose_op_apply_tiled(ops, dtype, n_states, &tiled);
+    fp_status st = fprt::choose_op_apply_tiled(ops, dtype, n_states, &tiled, summed);
     if (st != FP_OK)
         return st;
     uint64_t const state_bytes = uint64_t(dim) * n_states * (dtype == FP_C128 ? 16u : 8u);
@@ -663,7 +677,8 @@ extern "C" fp_status fp_hip_op_apply_plan(fp_hip_ops const *ops, int dtype, size
     }
     fprt::TileGeom const geo = fprt::tile_geom(dtype, ops->n_qubits, n_states, false);
     fprt::TileTables *tt = nullptr;
-    if ((st = fprt::tile_tables(ops, fprt::kTilesOpApply, geo.k, geo.dh, &tt)) != FP_OK)
+    if ((st = fprt::tile_tables(ops, summed ? fprt::kTilesSummedApply : fprt::kTilesOpApply, geo.k, geo.dh, &tt)) !=
+        FP_OK)
         return st;
     uint64_t const G = tt->n_groups;
     plan->kernel = FP_EV_TILED;
@@ -672,4 +687,18 @@ extern "C" fp_status fp_hip_op_apply_plan(fp_hip_ops const *ops, int dtype, size
     plan->states_per_block = 1u << geo.lsb;
     plan->hbm_bytes = state_bytes * (3u * G - 1u + out_reads);
     return FP_OK;
+}
+
+} // namespace
+
+extern "C" fp_status fp_hip_op_apply_plan(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_states, int accumulate,
+                                          fp_hip_ev_plan *plan)
+{
+    return apply_plan(ops, dtype, dim, n_states, accumulate, false, plan);
+}
+
+extern "C" fp_status fp_hip_summed_apply_plan(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_states,
+                                              int accumulate, fp_hip_ev_plan *plan)
+{
+    return apply_plan(ops, dtype, dim, n_states, accumulate, true, plan);
 }

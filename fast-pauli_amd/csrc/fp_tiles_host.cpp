@@ -373,7 +373,7 @@ fp_status build_tile_image(StringTable const &table, int kind, uint32_t k, uint3
 {
     StringTable const *ops = &table;
     std::vector<HostGroup> hgs;
-    fp_status const st = group_strings(ops, k, dh, kind == kTilesOpApply, hgs);
+    fp_status const st = group_strings(ops, k, dh, (kind == kTilesOpApply || kind == kTilesSummedApply), hgs);
     if (st != FP_OK)
         return st;
     std::vector<fpt::TileGroup> groups;
@@ -399,7 +399,7 @@ fp_status build_tile_image(StringTable const &table, int kind, uint32_t k, uint3
                                         ops->coeffs[2 * m.s + 1] * kap});
             }
         }
-        if (kind == kTilesOpApply)
+        if ((kind == kTilesOpApply || kind == kTilesSummedApply))
         {
             // one record per string, in the operator's order; n_chunks counts them
             for (LocalStr const &m : hg.mem)

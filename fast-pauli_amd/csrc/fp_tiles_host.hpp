@@ -28,7 +28,9 @@ enum
 {
     kTilesOpExpect = 0, // fpt::TileChunk / fpt::TileStr records (PauliOp expectation)
     kTilesEvMatrix = 1, // fpl::EvChunk / fpl::EvStr records (per-string EV matrix on cosets)
-    kTilesOpApply = 2   // fpa::ApplyStr records, groups of consecutive strings (PauliOp apply)
+    kTilesOpApply = 2,  // fpa::ApplyStr records, groups of consecutive strings (PauliOp apply)
+    kTilesSummedApply = 3 // the same for SummedPauliOp::apply: StringTable::cf128 / cf64 hold the
+                          // folded row sums of the coefficient matrix
 };
 
 // `expect`: tiles for the expectation kernel (complex128 takes 128-byte row slices), else apply
