@@ -252,14 +252,15 @@ __device__ __forceinline__ void chunk_rows_n(typename fpk::VecOf<T, 1>::type con
 {
     typedef typename fpk::RowAcc<T>::type A;
     typedef typename fpk::VecOf<T, 1>::type V;
-    constexpr bool SUMFIRST = true;
     uint32_t const x = ch.x;
     bool const pairs = ch.mode != kAllRows;
     uint32_t const hh = ch.mode == kPairs ? ch.h - 6 : 0;
     uint32_t const lo_mask = (1u << hh) - 1;
     uint32_t const n_iter = pairs ? (n_blocks64 >> 1) : n_blocks64;
     uint32_t const lr = lane_row(ch.mode, ch.h, lane);
-    auto ins = [&](uint32_t q) { return pairs ? (((q >> hh) << (hh + 1)) | (q & lo_mask)) : q; };
+    // a zero bit inserted at hh (pairs) or q itself: q + (q & hmask), two scalar ops
+    uint32_t const hmask = pairs ? ~lo_mask : 0u;
+    auto ins = [&](uint32_t q) { return q + (q & hmask); };
     auto g_of = [&](V const &a, V const &b) {
         A const ar = A(a[0]), ai = A(a[1]);
         if constexpr (KIND == 0)
@@ -300,37 +301,23 @@ __device__ __forceinline__ void chunk_rows_n(typename fpk::VecOf<T, 1>::type con
 #pragma unroll
         for (int s = 0; s < CNT; ++s)
             flip[s] = uint32_t(__popc(uq0 & zhi[s]) & 1) << 31;
-        if constexpr (SUMFIRST)
+        // the U terms of a string first, with its per-chunk signs, then this step's
+        // sign once: one scalar sign per string instead of one per term
+        A tsum[CNT];
+#pragma unroll
+        for (int s = 0; s < CNT; ++s)
+            tsum[s] = A(0);
+#pragma unroll
+        for (int k = 0; k < U; ++k)
         {
-            // the U terms of a string first, with its per-chunk signs, then this step's
-            // sign once: one scalar sign per string instead of one per term
-            A tsum[CNT];
+            A const g = g_of(a[k], b[k]);
 #pragma unroll
             for (int s = 0; s < CNT; ++s)
-                tsum[s] = A(0);
-#pragma unroll
-            for (int k = 0; k < U; ++k)
-            {
-                A const g = g_of(a[k], b[k]);
-#pragma unroll
-                for (int s = 0; s < CNT; ++s)
-                    tsum[s] = fma(sgk[s][k], g, tsum[s]);
-            }
-#pragma unroll
-            for (int s = 0; s < CNT; ++s)
-                acc[s] = fma(xor_sign(A(1), flip[s]), tsum[s], acc[s]);
+                tsum[s] = fma(sgk[s][k], g, tsum[s]);
         }
-        else
-        {
 #pragma unroll
-            for (int k = 0; k < U; ++k)
-            {
-                A const g = g_of(a[k], b[k]);
-#pragma unroll
-                for (int s = 0; s < CNT; ++s)
-                    acc[s] = fma(xor_sign(sgk[s][k], flip[s]), g, acc[s]);
-            }
-        }
+        for (int s = 0; s < CNT; ++s)
+            acc[s] = fma(xor_sign(A(1), flip[s]), tsum[s], acc[s]);
     }
     for (uint32_t q = n_full; q < n_iter; ++q) // fewer than U row blocks left
     {
