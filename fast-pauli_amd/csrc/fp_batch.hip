@@ -6,8 +6,10 @@
 // entry into work items, orders them (largest first by default), uploads the
 // descriptors, and captures [reset counters, persistent kernel] into a
 // hipGraph that is replayed on every launch.  Large apply entries whose string
-// groups share coset tiles become coset items (one string group, a run of 64 KiB
-// tiles each): about three state passes per group instead of one per string.
+// groups share coset tiles run as coset-tiled launches of their own (one per
+// string group, after the persistent kernel in the same graph) or become coset
+// items (one string group, a run of 64 KiB tiles each): about three state passes
+// per group instead of one per string.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,9 +40,13 @@ struct BatchEntry
 struct fp_hip_batch
 {
     std::vector<BatchEntry> entries;
-    // coset items by default for states of 256 MiB and more: below that the states
-    // stay in the Infinity Cache, whose repeated-read rate the streaming items reach
-    fp_hip_batch_schedule sched{2048, 1u << 18, 4, FP_BATCH_ORDER_LARGEST_FIRST, 8, 256u << 10};
+    // Coset-tiled launches of their own from 64 MiB of states (the single-operator rule);
+    // coset items would apply from 256 MiB (below that the states stay in the Infinity
+    // Cache, whose repeated-read rate the streaming items reach) but the launches take
+    // those entries first: in a 256-entry batch of complex64 applies, the 15 entries of
+    // 64 MiB and more took 33.4 ms as launches of their own, 38.9 ms as streaming items
+    // and 43.9 ms as coset items.
+    fp_hip_batch_schedule sched{2048, 1u << 18, 4, FP_BATCH_ORDER_LARGEST_FIRST, 8, 256u << 10, 64u << 10};
     bool use_graph = true;
     // device tables, sized by capacity and reused across rebuilds
     bool built = false;
@@ -59,6 +65,15 @@ struct fp_hip_batch
     bool coset = false;      // some entry runs as coset items: the 512-thread kernel
     size_t lds = 0;          // dynamic LDS per block
     size_t coset_entries = 0, coset_groups = 0, coset_items = 0;
+    // entries run as coset-tiled launches of their own (after the persistent kernel)
+    struct Split
+    {
+        uint32_t entry;
+        fprt::TileTables *tt;
+        fprt::TileGeom geo;
+    };
+    std::vector<Split> split;
+    size_t tiled_groups = 0;
     double plan_bytes = 0.0; // state passes of the plan: coset groups, or one per string
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -147,19 +162,34 @@ int device_cus()
     return cus;
 }
 
-void enqueue_kernels(fp_hip_batch *b, hipStream_t stream)
+fp_status enqueue_kernels(fp_hip_batch *b, hipStream_t stream)
 {
     uint32_t const n = b->n_done;
-    hipLaunchKernelGGL(fpb::k_batch_reset, dim3((n + 1 + fpk::kBlock - 1) / fpk::kBlock), dim3(fpk::kBlock), 0,
-                       stream, b->d_counter, b->d_counter + 1, n);
-    if (b->coset)
-        hipLaunchKernelGGL(fpb::k_batch_persistent<fpb::kTiledBlock>, dim3(b->n_blocks), dim3(fpb::kTiledBlock),
-                           b->lds, stream, b->d_ops, b->d_items, b->n_items, b->d_counter, b->d_counter + 1,
-                           b->d_partials, b->d_counter + 1 + n);
-    else
-        hipLaunchKernelGGL(fpb::k_batch_persistent<fpk::kBlock>, dim3(b->n_blocks), dim3(fpk::kBlock), b->lds,
-                           stream, b->d_ops, b->d_items, b->n_items, b->d_counter, b->d_counter + 1, b->d_partials,
-                           b->d_counter + 1 + n);
+    if (b->n_items > 0)
+    {
+        hipLaunchKernelGGL(fpb::k_batch_reset, dim3((n + 1 + fpk::kBlock - 1) / fpk::kBlock), dim3(fpk::kBlock), 0,
+                           stream, b->d_counter, b->d_counter + 1, n);
+        if (b->coset)
+            hipLaunchKernelGGL(fpb::k_batch_persistent<fpb::kTiledBlock>, dim3(b->n_blocks), dim3(fpb::kTiledBlock),
+                               b->lds, stream, b->d_ops, b->d_items, b->n_items, b->d_counter, b->d_counter + 1,
+                               b->d_partials, b->d_counter + 1 + n);
+        else
+            hipLaunchKernelGGL(fpb::k_batch_persistent<fpk::kBlock>, dim3(b->n_blocks), dim3(fpk::kBlock), b->lds,
+                               stream, b->d_ops, b->d_items, b->n_items, b->d_counter, b->d_counter + 1,
+                               b->d_partials, b->d_counter + 1 + n);
+        FP_LAUNCHED();
+    }
+    // (a second stream beside the persistent kernel gained nothing: 64.96 against 64.97 ms
+    // for the 256-entry complex64 batch)
+    for (fp_hip_batch::Split const &sp : b->split)
+    {
+        BatchEntry const &e = b->entries[sp.entry];
+        fp_status const r = fprt::launch_op_apply_tiled(sp.tt, sp.geo, e.dtype, e.states, e.out, e.dim, e.n_states,
+                                                        e.accumulate, stream);
+        if (r != FP_OK)
+            return r;
+    }
+    return FP_OK;
 }
 
 // Persistent blocks per CU that can be resident for the kernel the batch launches.
@@ -224,6 +254,8 @@ fp_status build(fp_hip_batch *b, hipStream_t stream)
     if (b->entries.empty())
         return fail(FP_ERR_INVALID_ARGUMENT, "batch is empty");
     b->coset_entries = b->coset_groups = b->coset_items = 0;
+    b->split.clear();
+    b->tiled_groups = 0;
     b->plan_bytes = 0.0;
     std::vector<fpb::BatchOp> ops;
     struct Item
@@ -275,6 +307,30 @@ fp_status build(fp_hip_batch *b, hipStream_t stream)
         op.accumulate = e.accumulate ? 1u : 0u;
         fprt::TileGeom geo{};
         fprt::TileTables *tt = nullptr;
+        if (e.kind == FP_BATCH_APPLY && sc.tiled_min_kib != 0)
+        {
+            fp_status ts = fprt::apply_tiled_plan(o, e.dtype, e.n_states, size_t(sc.tiled_min_kib) << 10, false, &geo,
+                                                  &tt);
+            if (ts == FP_OK && tt)
+                ts = fprt::check_tile_fit(tt, geo, e.dtype, fpa::kTileBytes);
+            if (ts == FP_OK && tt)
+                ts = fprt::tile_device(o, tt, stream);
+            if (ts == FP_OK && tt)
+                ts = fprt::apply_tiled_attributes();
+            if (ts != FP_OK)
+                return ts;
+            if (tt)
+            {
+                // launches of their own: no items in the persistent kernel
+                b->split.push_back(fp_hip_batch::Split{uint32_t(i), tt, geo});
+                b->tiled_groups += tt->n_groups;
+                b->plan_bytes += double(entry_in_bytes(e)) * (3.0 * tt->n_groups - (e.accumulate ? 0.0 : 1.0));
+                op.sshift = 64u;
+                op.n_tiles = 0;
+                ops.push_back(op);
+                continue;
+            }
+        }
         fp_status const cs = coset_plan(e, sc.coset_min_kib, &geo, &tt);
         if (cs != FP_OK)
             return cs;
@@ -476,9 +532,17 @@ fp_status build(fp_hip_batch *b, hipStream_t stream)
             FP_HIP(hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking));
         FP_HIP(hipStreamSynchronize(stream)); // tables uploaded before the graph can run on any stream
         FP_HIP(hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal));
-        enqueue_kernels(b, b->capture_stream);
+        fp_status const es = enqueue_kernels(b, b->capture_stream);
         hipError_t const le = hipGetLastError();
         hipError_t const ce = hipStreamEndCapture(b->capture_stream, &b->graph);
+        if (es != FP_OK || le != hipSuccess || ce != hipSuccess)
+        {
+            if (b->graph)
+                (void)hipGraphDestroy(b->graph);
+            b->graph = nullptr;
+        }
+        if (es != FP_OK)
+            return es;
         if (le != hipSuccess)
             return fail(FP_ERR_RUNTIME, "batch kernel capture failed: %s", hipGetErrorString(le));
         if (ce != hipSuccess)
@@ -511,6 +575,7 @@ fp_status fp_hip_batch_destroy(fp_hip_batch *b)
     release(b);
     if (b->capture_stream)
         (void)hipStreamDestroy(b->capture_stream);
+
     if (b->last_done)
         (void)hipEventDestroy(b->last_done);
     delete b;
@@ -631,6 +696,17 @@ fp_status fp_hip_batch_coset_info(fp_hip_batch const *b, size_t *n_entries, size
     return FP_OK;
 }
 
+fp_status fp_hip_batch_tiled_info(fp_hip_batch const *b, size_t *n_entries, size_t *n_groups)
+{
+    if (!b)
+        return fail(FP_ERR_INVALID_ARGUMENT, "batch is NULL");
+    if (n_entries)
+        *n_entries = b->split.size();
+    if (n_groups)
+        *n_groups = b->tiled_groups;
+    return FP_OK;
+}
+
 fp_status fp_hip_batch_check(fp_hip_batch *b)
 {
     if (!b)
@@ -668,8 +744,9 @@ fp_status fp_hip_batch_launch(fp_hip_batch *b, void *stream)
         FP_HIP(hipGraphLaunch(b->exec, st));
     else
     {
-        enqueue_kernels(b, st);
-        FP_LAUNCHED();
+        fp_status const r = enqueue_kernels(b, st);
+        if (r != FP_OK)
+            return r;
     }
     if (!b->last_done)
         FP_HIP(hipEventCreateWithFlags(&b->last_done, hipEventDisableTiming));
@@ -678,7 +755,7 @@ fp_status fp_hip_batch_launch(fp_hip_batch *b, void *stream)
 }
 
 // Coordinate descent over the schedule: tile size, expectation-tile work,
-// blocks per CU, coset-item threshold and length, then tile order; each candidate is timed over `reps` graph
+// blocks per CU, tiled-launch threshold, coset-item threshold and length, then tile order; each candidate is timed over `reps` graph
 // launches (after one warm-up) and the fastest is kept.  Outputs are
 // overwritten by the trial launches (accumulating entries accumulate reps+1
 // times), so callers tune on scratch outputs or re-initialise them.
@@ -739,6 +816,8 @@ fp_status fp_hip_batch_autotune(fp_hip_batch *b, void *stream, int reps, fp_hip_
                            {uint64_t(1) << 14, uint64_t(1) << 16, uint64_t(1) << 18, uint64_t(1) << 20})) == FP_OK &&
             (r = try_field([](fp_hip_batch_schedule &s, uint64_t v) { s.blocks_per_cu = uint32_t(v); },
                            {1, 2, 4, 8})) == FP_OK &&
+            (r = try_field([](fp_hip_batch_schedule &s, uint64_t v) { s.tiled_min_kib = uint32_t(v); },
+                           {0, 16384, 65536, 262144})) == FP_OK &&
             (r = try_field([](fp_hip_batch_schedule &s, uint64_t v) { s.coset_min_kib = uint32_t(v); },
                            {0, 4096, 32768, 262144})) == FP_OK &&
             (r = try_field([](fp_hip_batch_schedule &s, uint64_t v) { s.coset_tiles = uint32_t(v); },

@@ -129,11 +129,25 @@ fp_status run_ev_coset(fp_hip_ops const *ops, int dtype, void const *states, voi
                        void *scratch, hipStream_t st);
 void release_tiles(fp_hip_ops *ops);
 fp_status choose_op_expect_tiled(fp_hip_ops const *ops, int dtype, size_t n_states, bool *use_tiled);
-// `summed`: SummedPauliOp::apply (the row-summed coefficients) instead of PauliOp::apply
+// Coset-tiled apply: `tt` (and its geometry) when the state has at least `min_bytes`,
+// the set at least 8 strings, and its string groups' traffic (about three state passes
+// each) is at most 0.8 of the streaming form's (one per string); else `tt` = null.
+// `summed`: SummedPauliOp::apply (the row-summed coefficients) instead of PauliOp::apply.
+fp_status apply_tiled_plan(fp_hip_ops const *ops, int dtype, size_t n_states, size_t min_bytes, bool summed,
+                           TileGeom *geo, TileTables **tt);
+// The single-operator entry points take the tiled form from 64 MiB of states: below
+// that the caches serve the streaming kernel's repeated row reads.
+constexpr size_t kApplyTiledMinBytes = size_t(64) << 20;
 fp_status choose_op_apply_tiled(fp_hip_ops const *ops, int dtype, size_t n_states, bool *use_tiled,
                                 bool summed = false);
 fp_status run_op_apply_tiled(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim,
                              size_t n_states, int accumulate, hipStream_t st, bool summed = false);
+// The launches of run_op_apply_tiled (one per string group) on tables already on the
+// device; enqueues nothing else, so a graph capture may record it.
+fp_status launch_op_apply_tiled(TileTables const *tt, TileGeom const &geo, int dtype, void const *states, void *out,
+                                size_t dim, size_t n_states, int accumulate, hipStream_t st);
+// Sets the tiled apply kernels' LDS limit (once per process).
+fp_status apply_tiled_attributes();
 // With `ws`, the string-group tables and the partials go into the workspace (no
 // allocation; fails with FP_ERR_INVALID_ARGUMENT when they cannot fit), else the
 // tables' device copy and the stream's scratch.

@@ -392,57 +392,47 @@ fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *stat
 // PauliOp::apply on coset tiles (groups of consecutive strings, one launch each)
 // ---------------------------------------------------------------------------
 
+fp_status apply_tiled_plan(fp_hip_ops const *ops, int dtype, size_t n_states, size_t min_bytes, bool summed,
+                           TileGeom *geo, TileTables **out)
+{
+    *out = nullptr;
+    size_t const state_bytes = (size_t(1) << ops->n_qubits) * n_states * (dtype == FP_C128 ? 16 : 8);
+    if (ops->n_strings < 8 || state_bytes < min_bytes)
+        return FP_OK;
+    TileGeom const g = tile_geom(dtype, ops->n_qubits, n_states, false);
+    TileTables *tt = nullptr;
+    fp_status const st = tile_tables(ops, summed ? kTilesSummedApply : kTilesOpApply, g.k, g.dh, &tt);
+    if (st != FP_OK)
+        return st;
+    if (3.0 * double(tt->n_groups) > 0.8 * double(ops->n_strings))
+        return FP_OK;
+    *geo = g;
+    *out = tt;
+    return FP_OK;
+}
+
 // 0: streaming kernel, 1: tiled kernel, per FAST_PAULI_APPLY_KERNEL ("stream",
-// "tiled") or, by default, when the state is large (64 MiB and up: beyond the
-// caches that serve the streaming kernel's repeated row reads) and the groups'
-// traffic (about three state passes each) is below the strings' (one each).
+// "tiled") or, by default, by apply_tiled_plan with kApplyTiledMinBytes.
 fp_status choose_op_apply_tiled(fp_hip_ops const *ops, int dtype, size_t n_states, bool *use_tiled, bool summed)
 {
     *use_tiled = false;
     char const *v = std::getenv("FAST_PAULI_APPLY_KERNEL");
     if (v && std::strcmp(v, "stream") == 0)
         return FP_OK;
-    bool const forced = v && std::strcmp(v, "tiled") == 0;
-    size_t const state_bytes = (size_t(1) << ops->n_qubits) * n_states * (dtype == FP_C128 ? 16 : 8);
-    if (!forced && (ops->n_strings < 8 || state_bytes < (size_t(64) << 20)))
+    if (v && std::strcmp(v, "tiled") == 0)
+    {
+        *use_tiled = true;
         return FP_OK;
-    TileGeom const geo = tile_geom(dtype, ops->n_qubits, n_states, false);
+    }
+    TileGeom geo{};
     TileTables *tt = nullptr;
-    fp_status const st = tile_tables(ops, summed ? kTilesSummedApply : kTilesOpApply, geo.k, geo.dh, &tt);
-    if (st != FP_OK)
-        return st;
-    *use_tiled = forced || 3.0 * double(tt->n_groups) <= 0.8 * double(ops->n_strings);
-    return FP_OK;
+    fp_status const st = apply_tiled_plan(ops, dtype, n_states, kApplyTiledMinBytes, summed, &geo, &tt);
+    *use_tiled = tt != nullptr;
+    return st;
 }
 
-fp_status run_op_apply_tiled(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim,
-                             size_t n_states, int accumulate, hipStream_t st, bool summed)
+fp_status apply_tiled_attributes()
 {
-    TileGeom const geo = tile_geom(dtype, ops->n_qubits, n_states, false);
-    TileTables *tt = nullptr;
-    fp_status s = tile_tables(ops, summed ? kTilesSummedApply : kTilesOpApply, geo.k, geo.dh, &tt);
-    if (s != FP_OK)
-        return s;
-    if ((s = check_tile_fit(tt, geo, dtype, fpa::kTileBytes)) != FP_OK)
-        return s;
-    if ((s = tile_device(ops, tt, st)) != FP_OK)
-        return s;
-    uint64_t const n_cosets = uint64_t(dim) >> geo.d;
-    uint32_t const chunks_y = uint32_t((n_states + (size_t(1) << geo.lsb) - 1) >> geo.lsb);
-    if (chunks_y > 65535u)
-        return fail(FP_ERR_INVALID_ARGUMENT, "too many states for the tiled apply kernel");
-    // about four tiles per block (a block with one tile has nothing to overlap its
-    // loads with), between 1024 and 8192 blocks; FAST_PAULI_TILE_BLOCKS overrides
-    uint64_t const tiles = n_cosets * chunks_y;
-    uint64_t const target = env_u32("FAST_PAULI_TILE_BLOCKS", uint32_t(std::clamp<uint64_t>(tiles / 4, 1024, 8192)));
-    uint64_t const blocks_x = std::max<uint64_t>(1, std::min<uint64_t>(n_cosets, (target + chunks_y - 1) / chunks_y));
-    uint32_t const sshift = (n_states & (n_states - 1)) == 0 ? log2u(n_states) : 64u;
-    bool const wide = (uint64_t(dim) * n_states) >> 32 != 0 || env_u32("FAST_PAULI_TILE_WIDE", 0) == 1;
-    // complex64 full tiles with an even state count: two states per slot (16-byte
-    // loads, stores and LDS reads); a slot's two states are both live or both not
-    bool const aligned16 = ((reinterpret_cast<uintptr_t>(states) | reinterpret_cast<uintptr_t>(out)) & 15u) == 0;
-    bool const paired = dtype == FP_C64 && geo.lsb >= 1 && n_states % 2 == 0 && aligned16 &&
-                        geo.d + geo.lsb == log2u(fpa::kTileBytes / 8u);
     static std::once_flag attr_once;
     static hipError_t attr_err = hipSuccess;
     std::call_once(attr_once, [] {
@@ -461,6 +451,31 @@ fp_status run_op_apply_tiled(fp_hip_ops const *ops, int dtype, void const *state
     });
     if (attr_err != hipSuccess)
         return fail(FP_ERR_RUNTIME, "hipFuncSetAttribute failed: %s", hipGetErrorString(attr_err));
+    return FP_OK;
+}
+
+fp_status launch_op_apply_tiled(TileTables const *tt, TileGeom const &geo, int dtype, void const *states, void *out,
+                                size_t dim, size_t n_states, int accumulate, hipStream_t st)
+{
+    uint64_t const n_cosets = uint64_t(dim) >> geo.d;
+    uint32_t const chunks_y = uint32_t((n_states + (size_t(1) << geo.lsb) - 1) >> geo.lsb);
+    if (chunks_y > 65535u)
+        return fail(FP_ERR_INVALID_ARGUMENT, "too many states for the tiled apply kernel");
+    // about four tiles per block (a block with one tile has nothing to overlap its
+    // loads with), between 1024 and 8192 blocks; FAST_PAULI_TILE_BLOCKS overrides
+    uint64_t const tiles = n_cosets * chunks_y;
+    uint64_t const target = env_u32("FAST_PAULI_TILE_BLOCKS", uint32_t(std::clamp<uint64_t>(tiles / 4, 1024, 8192)));
+    uint64_t const blocks_x = std::max<uint64_t>(1, std::min<uint64_t>(n_cosets, (target + chunks_y - 1) / chunks_y));
+    uint32_t const sshift = (n_states & (n_states - 1)) == 0 ? log2u(n_states) : 64u;
+    bool const wide = (uint64_t(dim) * n_states) >> 32 != 0 || env_u32("FAST_PAULI_TILE_WIDE", 0) == 1;
+    // complex64 full tiles with an even state count: two states per slot (16-byte
+    // loads, stores and LDS reads); a slot's two states are both live or both not
+    bool const aligned16 = ((reinterpret_cast<uintptr_t>(states) | reinterpret_cast<uintptr_t>(out)) & 15u) == 0;
+    bool const paired = dtype == FP_C64 && geo.lsb >= 1 && n_states % 2 == 0 && aligned16 &&
+                        geo.d + geo.lsb == log2u(fpa::kTileBytes / 8u);
+    fp_status const as = apply_tiled_attributes();
+    if (as != FP_OK)
+        return as;
     dim3 const grid(uint32_t(blocks_x), chunks_y);
     auto const *strs = static_cast<fpa::ApplyStr const *>(tt->strs);
     for (uint32_t g = 0; g < tt->n_groups; ++g)
@@ -489,6 +504,21 @@ fp_status run_op_apply_tiled(fp_hip_ops const *ops, int dtype, void const *state
         FP_LAUNCHED();
     }
     return FP_OK;
+}
+
+fp_status run_op_apply_tiled(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim,
+                             size_t n_states, int accumulate, hipStream_t st, bool summed)
+{
+    TileGeom const geo = tile_geom(dtype, ops->n_qubits, n_states, false);
+    TileTables *tt = nullptr;
+    fp_status s = tile_tables(ops, summed ? kTilesSummedApply : kTilesOpApply, geo.k, geo.dh, &tt);
+    if (s != FP_OK)
+        return s;
+    if ((s = check_tile_fit(tt, geo, dtype, fpa::kTileBytes)) != FP_OK)
+        return s;
+    if ((s = tile_device(ops, tt, st)) != FP_OK)
+        return s;
+    return launch_op_apply_tiled(tt, geo, dtype, states, out, dim, n_states, accumulate, st);
 }
 
 // ---------------------------------------------------------------------------

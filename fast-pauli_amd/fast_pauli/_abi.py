@@ -140,7 +140,8 @@ class BatchSchedule(ctypes.Structure):
     """fp_hip_batch_schedule"""
     _fields_ = [("tile_vectors", ctypes.c_uint32), ("ev_tile_work", ctypes.c_uint64),
                 ("blocks_per_cu", ctypes.c_uint32), ("order", ctypes.c_int),
-                ("coset_tiles", ctypes.c_uint32), ("coset_min_kib", ctypes.c_uint32)]
+                ("coset_tiles", ctypes.c_uint32), ("coset_min_kib", ctypes.c_uint32),
+                ("tiled_min_kib", ctypes.c_uint32)]
 
     def as_dict(self) -> dict:
         return {f: int(getattr(self, f)) for f, _ in self._fields_}
@@ -156,6 +157,7 @@ _SIGS.update({
     "fp_hip_batch_info": [_vp, ctypes.POINTER(_sz), ctypes.POINTER(_sz), ctypes.POINTER(_sz)],
     "fp_hip_batch_coset_info": [_vp, ctypes.POINTER(_sz), ctypes.POINTER(_sz), ctypes.POINTER(_sz),
                                 ctypes.POINTER(ctypes.c_double)],
+    "fp_hip_batch_tiled_info": [_vp, ctypes.POINTER(_sz), ctypes.POINTER(_sz)],
     "fp_hip_batch_check": [_vp],
     "fp_hip_batch_launch": [_vp, _vp],
     "fp_hip_batch_set_graph": [_vp, _i],

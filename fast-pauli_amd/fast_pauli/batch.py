@@ -80,9 +80,11 @@ class Batch:
         pb = ctypes.c_double()
         _abi.call("fp_hip_batch_coset_info", self.handle, ctypes.byref(e), ctypes.byref(g), ctypes.byref(i),
                   ctypes.byref(pb))
+        te, tg = ctypes.c_size_t(), ctypes.c_size_t()
+        _abi.call("fp_hip_batch_tiled_info", self.handle, ctypes.byref(te), ctypes.byref(tg))
         return {"entries": a.value, "tiles": b.value, "blocks": c.value,
                 "coset_entries": e.value, "coset_groups": g.value, "coset_items": i.value,
-                "plan_bytes": pb.value}
+                "tiled_entries": te.value, "tiled_groups": tg.value, "plan_bytes": pb.value}
 
     def check(self) -> None:
         """Waits for the last launch; raises if a coset item gave up waiting for its

@@ -311,6 +311,10 @@ typedef struct fp_hip_batch_schedule
     uint32_t coset_min_kib; /* apply entries with states of at least this many KiB run as
                                coset items (string groups sharing LDS tiles) when that moves
                                fewer bytes; 0: never */
+    uint32_t tiled_min_kib; /* apply entries with states of at least this many KiB whose string
+                               groups pay (as in fp_hip_op_apply) run as coset-tiled launches of
+                               their own, one per group, after the persistent kernel in the same
+                               graph; they take precedence over coset items; 0: never */
 } fp_hip_batch_schedule;
 
 fp_status fp_hip_batch_create(fp_hip_batch **batch);
@@ -327,6 +331,9 @@ fp_status fp_hip_batch_info(fp_hip_batch const *batch, size_t *n_entries, size_t
  * expectation: one state pass per string). */
 fp_status fp_hip_batch_coset_info(fp_hip_batch const *batch, size_t *n_entries, size_t *n_groups, size_t *n_items,
                                   double *plan_bytes);
+/* Of the last build: entries run as coset-tiled launches of their own, and their string
+ * groups (one launch each). */
+fp_status fp_hip_batch_tiled_info(fp_hip_batch const *batch, size_t *n_entries, size_t *n_groups);
 /* Waits for the batch's last launch; FP_ERR_RUNTIME if a coset item of any launch since the
  * last build gave up waiting for the previous string group (an internal error: its outputs
  * are then unspecified). */

@@ -277,3 +277,59 @@ def test_coset_items_bit_exact(hip, graph):
         if e.kind == _abi.FP_BATCH_APPLY:
             assert np.array_equal(e.result(), r)
     b.destroy()
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_tiled_launch_entries_bit_exact(hip, graph):
+    """Apply entries over the tiled-launch threshold run as coset-tiled launches of their
+    own (one per string group, after the persistent kernel, in the same graph): IEEE-equal
+    to the operator alone, beside coset items and streaming entries, over repeated
+    launches; a batch of such entries alone launches no persistent kernel."""
+    H.say(f"batch tiled launches graph={graph}")
+    entries = coset_entries()
+    b = Batch()
+    if not graph:
+        _abi.call("fp_hip_batch_set_graph", b.handle, 0)
+    for e in entries:
+        b.add(e.kind, e.ops, e.d_in.ptr, e.d_out.ptr, 1 << e.n, e.S, e.dtype, e.accumulate)
+    refs = [e.alone() for e in entries]
+    # 3000 KiB: the two 2^13-row entries with 58 and 59 states launch on their own, the
+    # other large entries run as coset items
+    for sched in ({"tiled_min_kib": 1, "coset_min_kib": 0}, {"tiled_min_kib": 1, "coset_min_kib": 1},
+                  {"tiled_min_kib": 3000, "coset_min_kib": 1}):
+        b.schedule = sched
+        for rep in range(2):
+            for e in entries:
+                e.reset()
+            b.launch()
+            _abi.synchronize()
+            info = b.info()
+            assert info["tiled_entries"] >= 2, info
+            if sched["tiled_min_kib"] == 3000:
+                assert info["coset_entries"] >= 1, info
+            for e, r in zip(entries, refs):
+                got = e.result()
+                if e.kind == _abi.FP_BATCH_APPLY:
+                    assert np.array_equal(got, r), (sched, rep, e.n, e.N, e.S, e.dtype, e.accumulate)
+                else:
+                    tol = 1e-12 if e.dtype == C128 else 1e-5
+                    assert H.rel_err(got - e.base, r - e.base) <= tol
+            b.check()
+    b.destroy()
+    big = [entries[1], entries[10]]
+    b = Batch()
+    if not graph:
+        _abi.call("fp_hip_batch_set_graph", b.handle, 0)
+    for e in big:
+        b.add(e.kind, e.ops, e.d_in.ptr, e.d_out.ptr, 1 << e.n, e.S, e.dtype, e.accumulate)
+    b.schedule = {"tiled_min_kib": 1}
+    for rep in range(2):
+        for e in big:
+            e.reset()
+        b.launch()
+        _abi.synchronize()
+        info = b.info()
+        assert info["tiled_entries"] == 2 and info["tiles"] == 0, info
+        for e in big:
+            assert np.array_equal(e.result(), e.alone())
+    b.destroy()
