@@ -569,8 +569,9 @@ def bench_summed_n14(args) -> dict:
 
 def bench_summed_apply_n14(args, failures) -> dict:
     """SummedPauliOp::apply and apply_weighted on the n=14 operator set (2000 strings of
-    weight <= 3, 200 operators) and 1000 states, complex128, streaming kernels: every
-    output element sums its 2000 strings' terms in order (one partner read per string)."""
+    weight <= 3, 200 operators) and 1000 states, complex128: every output element sums its
+    2000 strings' terms in order, through the coset-tiled kernel (string groups sharing LDS
+    tiles) where its plan pays, else the streaming kernels (one partner read per string)."""
     from fast_pauli import _abi, synthetic as syn
 
     n, N, K, S = 14, 2000, 200, 1000
@@ -593,6 +594,7 @@ def bench_summed_apply_n14(args, failures) -> dict:
     state_bytes = dim * S * 16
     model = (N + 1) * state_bytes  # streaming: one partner read per string, one output write
     plan = _abi.summed_apply_plan(ops, _abi.FP_C128, dim, S)
+    wplan = _abi.summed_apply_weighted_plan(ops, _abi.FP_C128, dim, S)
     res = {"config": "n=14, 2000 strings of weight <= 3, h (2000 x 200) ~ N(0,1) real, x (200 x 1000) ~ N(0,1), "
                      "1000 normalised states, complex128",
            "apply_ms": ms_apply, "apply_weighted_ms": ms_w,
@@ -601,7 +603,9 @@ def bench_summed_apply_n14(args, failures) -> dict:
            "apply_plan_gbs": plan["hbm_bytes"] / (ms_apply / 1e3) / 1e9,
            "apply_plan_frac_hbm": plan["hbm_bytes"] / (ms_apply / 1e3) / 1e9 / HBM_PEAK_GBS,
            "streaming_model_bytes": model, "streaming_bytes_rule": "(strings + 1) x 2^n x S x 16 B",
-           "apply_weighted_gbs": model / (ms_w / 1e3) / 1e9,
+           "apply_weighted_kernel": "coset-tiled" if wplan["kernel"] == _abi.FP_EV_TILED else "streaming",
+           "apply_weighted_plan_bytes": wplan["hbm_bytes"],
+           "apply_weighted_plan_gbs": wplan["hbm_bytes"] / (ms_w / 1e3) / 1e9,
            "strings_x_states_per_s": N * S / (ms_apply / 1e3)}
     if not args.no_cpu:
         from oracle import oracle as O

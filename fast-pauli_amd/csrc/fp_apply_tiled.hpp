@@ -43,14 +43,21 @@ constexpr uint32_t kTileBytes = 64u * 1024u;
 // gridDim.x; state chunk blockIdx.y.  PAIRED requires complex64, S_blk >= 2, an
 // even state count (so a slot's two states are both live or both not) and
 // 2^d rows x S_blk = one full tile.
-template <class T, bool WIDE, bool PAIRED>
+// WEIGHTED (SummedPauliOp::apply_weighted): string s of state t has the coefficient
+// (-i)^{nY_s} w(s, t), with w the (strings x states) complex table `wts` and the
+// records' coefficient the unit (-i)^{nY_s}.  The unit is applied to w by exact
+// negation and swap, and the product with the partner value uses the same four
+// products and two sums as (-i)^{nY} psi times w, so the terms are IEEE-equal to
+// the sequential loop's.
+template <class T, bool WIDE, bool PAIRED, bool WEIGHTED>
 __global__ __launch_bounds__(kThreads, 4) void k_op_apply_tiled(typename fpk::VecOf<T, 1>::type const *__restrict__ states,
                                                              typename fpk::VecOf<T, 1>::type *__restrict__ out,
                                                              uint64_t n_states, uint32_t sshift, uint32_t lsb,
                                                              uint32_t k, uint32_t d, uint64_t n_cosets,
                                                              fpt::TileGroup const *__restrict__ group,
                                                              uint64_t const *__restrict__ btab,
-                                                             ApplyStr const *__restrict__ strs, int init_zero)
+                                                             ApplyStr const *__restrict__ strs, int init_zero,
+                                                             T const *__restrict__ wts)
 {
     static_assert(!PAIRED || sizeof(T) == 4, "two states per slot: complex64 only");
     typedef typename fpk::VecOf<T, 1>::type V;
@@ -165,8 +172,26 @@ __global__ __launch_bounds__(kThreads, 4) void k_op_apply_tiled(typename fpk::Ve
                 tile[e] = reg[m];
         }
     };
+    // WEIGHTED: this thread's weights w(s, t) of string s (zero for dead states)
+    auto load_w = [&](uint32_t s, T (&wr)[NS], T (&wi)[NS]) {
+#pragma unroll
+        for (uint32_t j = 0; j < NS; ++j)
+        {
+            uint64_t const t = s0 + t0 + j;
+            wr[j] = T(0);
+            wi[j] = T(0);
+            if (t < n_states)
+            {
+                wr[j] = wts[2 * (uint64_t(gs[s].s) * n_states + t)];
+                wi[j] = wts[2 * (uint64_t(gs[s].s) * n_states + t) + 1];
+            }
+        }
+    };
     // the group's strings, in order, on the tile in LDS
     auto compute = [&](uint64_t q, LV(&acc)[EPT]) {
+        T nwr[NS], nwi[NS]; // WEIGHTED: the next string's weights, requested one string ahead
+        if constexpr (WEIGHTED)
+            load_w(0, nwr, nwi);
         for (uint32_t s = 0; s < n_str; ++s)
         {
             ApplyStr const st = gs[s];
@@ -186,19 +211,57 @@ __global__ __launch_bounds__(kThreads, 4) void k_op_apply_tiled(typename fpk::Ve
             unsigned char const *pl = smem + ((((a_lane ^ A_lane) << lsbx) | tx) * kLsz);
             // lane sign folded into the coefficient (negation is exact)
             bool const lneg = ((uint32_t(__popc(a_lane & st.Z)) ^ cs) & 1u) != 0;
-            T const lcr = lneg ? -cr : cr;
-            T const lci = lneg ? -ci : ci;
+            T lcr[NS], lci[NS];
+            if constexpr (WEIGHTED)
+            {
+                T wr[NS], wi[NS];
+#pragma unroll
+                for (uint32_t j = 0; j < NS; ++j)
+                {
+                    wr[j] = nwr[j];
+                    wi[j] = nwi[j];
+                }
+                if (s + 1 < n_str)
+                    load_w(s + 1, nwr, nwi);
+                // (cr, ci) = (-i)^{nY}: (1, 0), (0, -1), (-1, 0) or (0, 1)
+#pragma unroll
+                for (uint32_t j = 0; j < NS; ++j)
+                {
+                    T ur, ui;
+                    if (cr != T(0))
+                    {
+                        ur = cr > T(0) ? wr[j] : -wr[j];
+                        ui = cr > T(0) ? wi[j] : -wi[j];
+                    }
+                    else
+                    {
+                        ur = ci < T(0) ? wi[j] : -wi[j];
+                        ui = ci < T(0) ? -wr[j] : wr[j];
+                    }
+                    lcr[j] = lneg ? -ur : ur;
+                    lci[j] = lneg ? -ui : ui;
+                }
+            }
+            else
+            {
+#pragma unroll
+                for (uint32_t j = 0; j < NS; ++j)
+                {
+                    lcr[j] = lneg ? -cr : cr;
+                    lci[j] = lneg ? -ci : ci;
+                }
+            }
 #pragma unroll
             for (uint32_t m = 0; m < EPT; ++m)
             {
                 uint32_t const mrow = m << rm;
                 LV const src = *reinterpret_cast<LV const *>(pl + (((mrow ^ A_m) << lsbx) * kLsz));
                 uint32_t const mbit = uint32_t(__popc(mrow & st.Z) & 1u) << 31;
-                T const a = fpt::xor_sign_bit(lcr, mbit);
-                T const b = fpt::xor_sign_bit(lci, mbit);
 #pragma unroll
                 for (uint32_t j = 0; j < NS; ++j)
                 {
+                    T const a = fpt::xor_sign_bit(lcr[j], mbit);
+                    T const b = fpt::xor_sign_bit(lci[j], mbit);
                     T orr, oi;
                     fpk::cmul_exact<T>(a, b, src[2 * j], src[2 * j + 1], orr, oi);
                     acc[m][2 * j] = fpk::add_exact<T>(acc[m][2 * j], orr);
@@ -209,8 +272,9 @@ __global__ __launch_bounds__(kThreads, 4) void k_op_apply_tiled(typename fpk::Ve
     };
     // complex128 with 32-bit offsets has the registers to request the next coset's
     // output values with its state values (two output sets, alternating); other
-    // forms request a coset's output values just before its tile store
-    constexpr bool PF = sizeof(T) == 8 && !WIDE;
+    // forms (and the weighted one, whose weights take the registers) request a
+    // coset's output values just before its tile store
+    constexpr bool PF = sizeof(T) == 8 && !WIDE && !WEIGHTED;
     if constexpr (PF)
     {
         LV acc0[EPT], acc1[EPT];

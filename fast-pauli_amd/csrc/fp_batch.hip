@@ -309,8 +309,8 @@ fp_status build(fp_hip_batch *b, hipStream_t stream)
         fprt::TileTables *tt = nullptr;
         if (e.kind == FP_BATCH_APPLY && sc.tiled_min_kib != 0)
         {
-            fp_status ts = fprt::apply_tiled_plan(o, e.dtype, e.n_states, size_t(sc.tiled_min_kib) << 10, false, &geo,
-                                                  &tt);
+            fp_status ts = fprt::apply_tiled_plan(o, e.dtype, e.n_states, size_t(sc.tiled_min_kib) << 10,
+                                                  fprt::kTilesOpApply, &geo, &tt);
             if (ts == FP_OK && tt)
                 ts = fprt::check_tile_fit(tt, geo, e.dtype, fpa::kTileBytes);
             if (ts == FP_OK && tt)

@@ -1474,10 +1474,11 @@ fp_status fp_hip_summed_apply(fp_hip_ops const *ops, int dtype, void const *stat
     hipStream_t const s = static_cast<hipStream_t>(stream);
     // large states whose strings share coset tiles: the tiled kernel (same operations, same order)
     bool use_tiled = false;
-    if ((st = fprt::choose_op_apply_tiled(ops, dtype, n_states, &use_tiled, true)) != FP_OK)
+    if ((st = fprt::choose_op_apply_tiled(ops, dtype, n_states, &use_tiled, fprt::kTilesSummedApply)) != FP_OK)
         return st;
     if (use_tiled)
-        return fprt::run_op_apply_tiled(ops, dtype, states, out, dim, n_states, accumulate, s, true);
+        return fprt::run_op_apply_tiled(ops, dtype, states, out, dim, n_states, accumulate, s,
+                                        fprt::kTilesSummedApply);
     VecGeom const g = vec_geom(dtype, dim, n_states, states, out);
     uint32_t const N = uint32_t(ops->n_strings);
     if (dtype == FP_C128)
@@ -1522,6 +1523,13 @@ fp_status fp_hip_summed_apply_weighted(fp_hip_ops const *ops, int dtype, void co
                            static_cast<float const *>(data), static_cast<float *>(scratch), uint64_t(N), uint64_t(K),
                            uint64_t(S));
     FP_LAUNCHED();
+    // large states whose string groups pay: the coset-tiled kernel with per-state weights
+    bool use_tiled = false;
+    if ((st = fprt::choose_op_apply_tiled(ops, dtype, n_states, &use_tiled, fprt::kTilesWeightedApply)) != FP_OK)
+        return st;
+    if (use_tiled)
+        return fprt::run_op_apply_tiled(ops, dtype, states, out, dim, n_states, accumulate, s,
+                                        fprt::kTilesWeightedApply, scratch);
     VecGeom const g = vec_geom(dtype, dim, n_states, states, out);
     bool const acc = accumulate != 0;
     // four rows per thread (one for a 1-qubit operator, whose 2 rows do not split)

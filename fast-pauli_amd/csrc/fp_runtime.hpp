@@ -132,20 +132,24 @@ fp_status choose_op_expect_tiled(fp_hip_ops const *ops, int dtype, size_t n_stat
 // Coset-tiled apply: `tt` (and its geometry) when the state has at least `min_bytes`,
 // the set at least 8 strings, and its string groups' traffic (about three state passes
 // each) is at most 0.8 of the streaming form's (one per string); else `tt` = null.
-// `summed`: SummedPauliOp::apply (the row-summed coefficients) instead of PauliOp::apply.
-fp_status apply_tiled_plan(fp_hip_ops const *ops, int dtype, size_t n_states, size_t min_bytes, bool summed,
+// `kind`: kTilesOpApply (PauliOp::apply), kTilesSummedApply (SummedPauliOp::apply, the
+// row-summed coefficients) or kTilesWeightedApply (SummedPauliOp::apply_weighted).
+fp_status apply_tiled_plan(fp_hip_ops const *ops, int dtype, size_t n_states, size_t min_bytes, int kind,
                            TileGeom *geo, TileTables **tt);
 // The single-operator entry points take the tiled form from 64 MiB of states: below
 // that the caches serve the streaming kernel's repeated row reads.
 constexpr size_t kApplyTiledMinBytes = size_t(64) << 20;
 fp_status choose_op_apply_tiled(fp_hip_ops const *ops, int dtype, size_t n_states, bool *use_tiled,
-                                bool summed = false);
+                                int kind = kTilesOpApply);
+// `weights` (kTilesWeightedApply only): the (strings x states) complex weights w(s, t)
 fp_status run_op_apply_tiled(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim,
-                             size_t n_states, int accumulate, hipStream_t st, bool summed = false);
+                             size_t n_states, int accumulate, hipStream_t st, int kind = kTilesOpApply,
+                             void const *weights = nullptr);
 // The launches of run_op_apply_tiled (one per string group) on tables already on the
 // device; enqueues nothing else, so a graph capture may record it.
 fp_status launch_op_apply_tiled(TileTables const *tt, TileGeom const &geo, int dtype, void const *states, void *out,
-                                size_t dim, size_t n_states, int accumulate, hipStream_t st);
+                                size_t dim, size_t n_states, int accumulate, hipStream_t st,
+                                void const *weights = nullptr);
 // Sets the tiled apply kernels' LDS limit (once per process).
 fp_status apply_tiled_attributes();
 // With `ws`, the string-group tables and the partials go into the workspace (no

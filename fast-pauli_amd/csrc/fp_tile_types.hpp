@@ -106,7 +106,8 @@ struct ApplyStr
     double ci;
     float fr;     // the same, complex64 (rounded as the streaming kernel's table)
     float fi;
-    uint32_t pad[2];
+    uint32_t s;   // the string's index in the operator
+    uint32_t pad;
 };
 
 } // namespace fpa

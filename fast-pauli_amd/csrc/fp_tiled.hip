@@ -56,14 +56,27 @@ fp_status tile_tables(fp_hip_ops const *ops, int kind, uint32_t k, uint32_t dh, 
     bool const rows = kind == kTilesSummedApply;
     if (rows && (ops->rowsum128.size() != 2 * ops->n_strings || ops->rowsum64.size() != 2 * ops->n_strings))
         return fail(FP_ERR_INVALID_ARGUMENT, "SummedPauliOp has no coefficient matrix");
-    StringTable const st{ops->n_qubits,
-                         ops->n_strings,
-                         ops->x.data(),
-                         ops->z.data(),
-                         ops->ny.data(),
-                         ops->coeffs.data(),
-                         rows ? ops->rowsum128.data() : ops->cf128.data(),
-                         rows ? ops->rowsum64.data() : ops->cf64.data()};
+    // weighted apply: the records carry (-i)^{nY} only
+    std::vector<double> unit128;
+    std::vector<float> unit64;
+    if (kind == kTilesWeightedApply)
+    {
+        unit128.resize(2 * ops->n_strings);
+        unit64.resize(2 * ops->n_strings);
+        for (size_t i = 0; i < ops->n_strings; ++i)
+        {
+            static double const re[4] = {1.0, 0.0, -1.0, 0.0}, im[4] = {0.0, -1.0, 0.0, 1.0};
+            uint32_t const ny = ops->ny[i] & 3u;
+            unit128[2 * i] = re[ny];
+            unit128[2 * i + 1] = im[ny];
+            unit64[2 * i] = float(re[ny]);
+            unit64[2 * i + 1] = float(im[ny]);
+        }
+    }
+    double const *c128 = rows ? ops->rowsum128.data() : (unit128.empty() ? ops->cf128.data() : unit128.data());
+    float const *c64 = rows ? ops->rowsum64.data() : (unit64.empty() ? ops->cf64.data() : unit64.data());
+    StringTable const st{ops->n_qubits, ops->n_strings, ops->x.data(), ops->z.data(), ops->ny.data(),
+                         ops->coeffs.data(), c128, c64};
     TileImage img;
     fp_status const bs = build_tile_image(st, kind, k, dh, wht_min, img);
     if (bs != FP_OK)
@@ -392,7 +405,7 @@ fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *stat
 // PauliOp::apply on coset tiles (groups of consecutive strings, one launch each)
 // ---------------------------------------------------------------------------
 
-fp_status apply_tiled_plan(fp_hip_ops const *ops, int dtype, size_t n_states, size_t min_bytes, bool summed,
+fp_status apply_tiled_plan(fp_hip_ops const *ops, int dtype, size_t n_states, size_t min_bytes, int kind,
                            TileGeom *geo, TileTables **out)
 {
     *out = nullptr;
@@ -401,7 +414,7 @@ fp_status apply_tiled_plan(fp_hip_ops const *ops, int dtype, size_t n_states, si
         return FP_OK;
     TileGeom const g = tile_geom(dtype, ops->n_qubits, n_states, false);
     TileTables *tt = nullptr;
-    fp_status const st = tile_tables(ops, summed ? kTilesSummedApply : kTilesOpApply, g.k, g.dh, &tt);
+    fp_status const st = tile_tables(ops, kind, g.k, g.dh, &tt);
     if (st != FP_OK)
         return st;
     if (3.0 * double(tt->n_groups) > 0.8 * double(ops->n_strings))
@@ -413,7 +426,7 @@ fp_status apply_tiled_plan(fp_hip_ops const *ops, int dtype, size_t n_states, si
 
 // 0: streaming kernel, 1: tiled kernel, per FAST_PAULI_APPLY_KERNEL ("stream",
 // "tiled") or, by default, by apply_tiled_plan with kApplyTiledMinBytes.
-fp_status choose_op_apply_tiled(fp_hip_ops const *ops, int dtype, size_t n_states, bool *use_tiled, bool summed)
+fp_status choose_op_apply_tiled(fp_hip_ops const *ops, int dtype, size_t n_states, bool *use_tiled, int kind)
 {
     *use_tiled = false;
     char const *v = std::getenv("FAST_PAULI_APPLY_KERNEL");
@@ -426,7 +439,7 @@ fp_status choose_op_apply_tiled(fp_hip_ops const *ops, int dtype, size_t n_state
     }
     TileGeom geo{};
     TileTables *tt = nullptr;
-    fp_status const st = apply_tiled_plan(ops, dtype, n_states, kApplyTiledMinBytes, summed, &geo, &tt);
+    fp_status const st = apply_tiled_plan(ops, dtype, n_states, kApplyTiledMinBytes, kind, &geo, &tt);
     *use_tiled = tt != nullptr;
     return st;
 }
@@ -436,12 +449,18 @@ fp_status apply_tiled_attributes()
     static std::once_flag attr_once;
     static hipError_t attr_err = hipSuccess;
     std::call_once(attr_once, [] {
-        void const *fns[] = {reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<double, false, false>),
-                             reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<double, true, false>),
-                             reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<float, false, false>),
-                             reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<float, true, false>),
-                             reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<float, false, true>),
-                             reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<float, true, true>)};
+        void const *fns[] = {reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<double, false, false, false>),
+                             reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<double, true, false, false>),
+                             reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<float, false, false, false>),
+                             reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<float, true, false, false>),
+                             reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<float, false, true, false>),
+                             reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<float, true, true, false>),
+                             reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<double, false, false, true>),
+                             reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<double, true, false, true>),
+                             reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<float, false, false, true>),
+                             reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<float, true, false, true>),
+                             reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<float, false, true, true>),
+                             reinterpret_cast<void const *>(&fpa::k_op_apply_tiled<float, true, true, true>)};
         for (void const *f : fns)
         {
             hipError_t const e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, int(fpa::kTileBytes));
@@ -455,7 +474,7 @@ fp_status apply_tiled_attributes()
 }
 
 fp_status launch_op_apply_tiled(TileTables const *tt, TileGeom const &geo, int dtype, void const *states, void *out,
-                                size_t dim, size_t n_states, int accumulate, hipStream_t st)
+                                size_t dim, size_t n_states, int accumulate, hipStream_t st, void const *weights)
 {
     uint64_t const n_cosets = uint64_t(dim) >> geo.d;
     uint32_t const chunks_y = uint32_t((n_states + (size_t(1) << geo.lsb) - 1) >> geo.lsb);
@@ -478,28 +497,36 @@ fp_status launch_op_apply_tiled(TileTables const *tt, TileGeom const &geo, int d
         return as;
     dim3 const grid(uint32_t(blocks_x), chunks_y);
     auto const *strs = static_cast<fpa::ApplyStr const *>(tt->strs);
+    bool const wtd = weights != nullptr;
     for (uint32_t g = 0; g < tt->n_groups; ++g)
     {
         int const init_zero = (g == 0 && !accumulate) ? 1 : 0;
         if (dtype == FP_C128)
         {
             typedef fpk::VecOf<double, 1>::type V;
-            auto const kfn =
-                wide ? &fpa::k_op_apply_tiled<double, true, false> : &fpa::k_op_apply_tiled<double, false, false>;
+            auto const kfn = wtd ? (wide ? &fpa::k_op_apply_tiled<double, true, false, true>
+                                         : &fpa::k_op_apply_tiled<double, false, false, true>)
+                                 : (wide ? &fpa::k_op_apply_tiled<double, true, false, false>
+                                         : &fpa::k_op_apply_tiled<double, false, false, false>);
             hipLaunchKernelGGL(kfn, grid, dim3(fpa::kThreads), fpa::kTileBytes, st, static_cast<V const *>(states),
                                static_cast<V *>(out), uint64_t(n_states), sshift, geo.lsb, geo.k, geo.d, n_cosets,
-                               tt->groups + g, tt->btab, strs, init_zero);
+                               tt->groups + g, tt->btab, strs, init_zero, static_cast<double const *>(weights));
         }
         else
         {
             typedef fpk::VecOf<float, 1>::type V;
-            auto const kfn = paired ? (wide ? &fpa::k_op_apply_tiled<float, true, true>
-                                            : &fpa::k_op_apply_tiled<float, false, true>)
-                                    : (wide ? &fpa::k_op_apply_tiled<float, true, false>
-                                            : &fpa::k_op_apply_tiled<float, false, false>);
+            auto const kfn =
+                wtd ? (paired ? (wide ? &fpa::k_op_apply_tiled<float, true, true, true>
+                                      : &fpa::k_op_apply_tiled<float, false, true, true>)
+                              : (wide ? &fpa::k_op_apply_tiled<float, true, false, true>
+                                      : &fpa::k_op_apply_tiled<float, false, false, true>))
+                    : (paired ? (wide ? &fpa::k_op_apply_tiled<float, true, true, false>
+                                      : &fpa::k_op_apply_tiled<float, false, true, false>)
+                              : (wide ? &fpa::k_op_apply_tiled<float, true, false, false>
+                                      : &fpa::k_op_apply_tiled<float, false, false, false>));
             hipLaunchKernelGGL(kfn, grid, dim3(fpa::kThreads), fpa::kTileBytes, st, static_cast<V const *>(states),
                                static_cast<V *>(out), uint64_t(n_states), sshift, geo.lsb, geo.k, geo.d, n_cosets,
-                               tt->groups + g, tt->btab, strs, init_zero);
+                               tt->groups + g, tt->btab, strs, init_zero, static_cast<float const *>(weights));
         }
         FP_LAUNCHED();
     }
@@ -507,18 +534,20 @@ fp_status launch_op_apply_tiled(TileTables const *tt, TileGeom const &geo, int d
 }
 
 fp_status run_op_apply_tiled(fp_hip_ops const *ops, int dtype, void const *states, void *out, size_t dim,
-                             size_t n_states, int accumulate, hipStream_t st, bool summed)
+                             size_t n_states, int accumulate, hipStream_t st, int kind, void const *weights)
 {
+    if ((kind == kTilesWeightedApply) != (weights != nullptr))
+        return fail(FP_ERR_INVALID_ARGUMENT, "weights go with the weighted tables only");
     TileGeom const geo = tile_geom(dtype, ops->n_qubits, n_states, false);
     TileTables *tt = nullptr;
-    fp_status s = tile_tables(ops, summed ? kTilesSummedApply : kTilesOpApply, geo.k, geo.dh, &tt);
+    fp_status s = tile_tables(ops, kind, geo.k, geo.dh, &tt);
     if (s != FP_OK)
         return s;
     if ((s = check_tile_fit(tt, geo, dtype, fpa::kTileBytes)) != FP_OK)
         return s;
     if ((s = tile_device(ops, tt, st)) != FP_OK)
         return s;
-    return launch_op_apply_tiled(tt, geo, dtype, states, out, dim, n_states, accumulate, st);
+    return launch_op_apply_tiled(tt, geo, dtype, states, out, dim, n_states, accumulate, st, weights);
 }
 
 // ---------------------------------------------------------------------------
@@ -679,7 +708,7 @@ extern "C" fp_status fp_hip_op_expectation_plan(fp_hip_ops const *ops, int dtype
 namespace
 {
 
-fp_status apply_plan(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_states, int accumulate, bool summed,
+fp_status apply_plan(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_states, int accumulate, int kind,
                      fp_hip_ev_plan *plan)
 {
     if (!ops || !plan)
@@ -689,10 +718,10 @@ fp_status apply_plan(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_stat
     if (n_states == 0 || dim != (size_t(1) << ops->n_qubits))
         return fail(FP_ERR_INVALID_ARGUMENT, "shape (%zu, %zu) does not match %zu qubits", dim, n_states,
                     ops->n_qubits);
-    if (summed && ops->n_operators == 0)
+    if (kind != fprt::kTilesOpApply && ops->n_operators == 0)
         return fail(FP_ERR_INVALID_ARGUMENT, "SummedPauliOp has no coefficient matrix");
     bool tiled = false;
-    fp_status st = fprt::choose_op_apply_tiled(ops, dtype, n_states, &tiled, summed);
+    fp_status st = fprt::choose_op_apply_tiled(ops, dtype, n_states, &tiled, kind);
     if (st != FP_OK)
         return st;
     uint64_t const state_bytes = uint64_t(dim) * n_states * (dtype == FP_C128 ? 16u : 8u);
@@ -707,8 +736,7 @@ fp_status apply_plan(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_stat
     }
     fprt::TileGeom const geo = fprt::tile_geom(dtype, ops->n_qubits, n_states, false);
     fprt::TileTables *tt = nullptr;
-    if ((st = fprt::tile_tables(ops, summed ? fprt::kTilesSummedApply : fprt::kTilesOpApply, geo.k, geo.dh, &tt)) !=
-        FP_OK)
+    if ((st = fprt::tile_tables(ops, kind, geo.k, geo.dh, &tt)) != FP_OK)
         return st;
     uint64_t const G = tt->n_groups;
     plan->kernel = FP_EV_TILED;
@@ -724,11 +752,17 @@ fp_status apply_plan(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_stat
 extern "C" fp_status fp_hip_op_apply_plan(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_states, int accumulate,
                                           fp_hip_ev_plan *plan)
 {
-    return apply_plan(ops, dtype, dim, n_states, accumulate, false, plan);
+    return apply_plan(ops, dtype, dim, n_states, accumulate, fprt::kTilesOpApply, plan);
 }
 
 extern "C" fp_status fp_hip_summed_apply_plan(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_states,
                                               int accumulate, fp_hip_ev_plan *plan)
 {
-    return apply_plan(ops, dtype, dim, n_states, accumulate, true, plan);
+    return apply_plan(ops, dtype, dim, n_states, accumulate, fprt::kTilesSummedApply, plan);
+}
+
+extern "C" fp_status fp_hip_summed_apply_weighted_plan(fp_hip_ops const *ops, int dtype, size_t dim,
+                                                       size_t n_states, int accumulate, fp_hip_ev_plan *plan)
+{
+    return apply_plan(ops, dtype, dim, n_states, accumulate, fprt::kTilesWeightedApply, plan);
 }

@@ -373,7 +373,7 @@ fp_status build_tile_image(StringTable const &table, int kind, uint32_t k, uint3
 {
     StringTable const *ops = &table;
     std::vector<HostGroup> hgs;
-    fp_status const st = group_strings(ops, k, dh, (kind == kTilesOpApply || kind == kTilesSummedApply), hgs);
+    fp_status const st = group_strings(ops, k, dh, apply_kind(kind), hgs);
     if (st != FP_OK)
         return st;
     std::vector<fpt::TileGroup> groups;
@@ -399,7 +399,7 @@ fp_status build_tile_image(StringTable const &table, int kind, uint32_t k, uint3
                                         ops->coeffs[2 * m.s + 1] * kap});
             }
         }
-        if ((kind == kTilesOpApply || kind == kTilesSummedApply))
+        if (apply_kind(kind))
         {
             // one record per string, in the operator's order; n_chunks counts them
             for (LocalStr const &m : hg.mem)
@@ -412,6 +412,7 @@ fp_status build_tile_image(StringTable const &table, int kind, uint32_t k, uint3
                 r.ci = ops->cf128[2 * m.s + 1];
                 r.fr = ops->cf64[2 * m.s];
                 r.fi = ops->cf64[2 * m.s + 1];
+                r.s = m.s;
                 strs.insert(strs.end(), reinterpret_cast<unsigned char const *>(&r),
                             reinterpret_cast<unsigned char const *>(&r) + sizeof(r));
                 ++n_chunks;

@@ -29,9 +29,17 @@ enum
     kTilesOpExpect = 0, // fpt::TileChunk / fpt::TileStr records (PauliOp expectation)
     kTilesEvMatrix = 1, // fpl::EvChunk / fpl::EvStr records (per-string EV matrix on cosets)
     kTilesOpApply = 2,  // fpa::ApplyStr records, groups of consecutive strings (PauliOp apply)
-    kTilesSummedApply = 3 // the same for SummedPauliOp::apply: StringTable::cf128 / cf64 hold the
-                          // folded row sums of the coefficient matrix
+    kTilesSummedApply = 3, // the same for SummedPauliOp::apply: StringTable::cf128 / cf64 hold the
+                           // folded row sums of the coefficient matrix
+    kTilesWeightedApply = 4 // the same for SummedPauliOp::apply_weighted: StringTable::cf128 / cf64
+                            // hold (-i)^{nY} (the per-state weights are applied in the kernel)
 };
+
+// The apply kinds: string groups of consecutive strings, one ApplyStr per string.
+inline bool apply_kind(int kind)
+{
+    return kind == kTilesOpApply || kind == kTilesSummedApply || kind == kTilesWeightedApply;
+}
 
 // `expect`: tiles for the expectation kernel (complex128 takes 128-byte row slices), else apply
 TileGeom tile_geom(int dtype, size_t n_qubits, size_t n_states, bool expect);

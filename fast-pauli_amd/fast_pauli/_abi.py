@@ -88,6 +88,7 @@ _SIGS = {
     "fp_hip_op_expectation_plan": [_vp, _i, _sz, _sz, _vp],
     "fp_hip_op_apply_plan": [_vp, _i, _sz, _sz, _i, _vp],
     "fp_hip_summed_apply_plan": [_vp, _i, _sz, _sz, _i, _vp],
+    "fp_hip_summed_apply_weighted_plan": [_vp, _i, _sz, _sz, _i, _vp],
     "fp_hip_op_expectation_noalloc": [_vp, _i, _vp, _vp, _sz, _sz, _i, _vp],
     "fp_hip_strings_expectation_matrix": [_vp, _i, _vp, _vp, _sz, _sz, _vp],
     "fp_hip_summed_expectation": [_vp, _i, _i, _vp, _vp, _sz, _sz, _i, _vp],
@@ -126,6 +127,14 @@ def op_apply_plan(ops, dtype: int, dim: int, n_states: int, accumulate: bool = F
     """Which kernel fp_hip_op_apply runs for these shapes, and its HBM bytes."""
     p = EvPlan()
     call("fp_hip_op_apply_plan", ops.handle, dtype, dim, n_states, int(bool(accumulate)), ctypes.byref(p))
+    return p.as_dict()
+
+
+def summed_apply_weighted_plan(ops, dtype: int, dim: int, n_states: int, accumulate: bool = False) -> dict:
+    """Which kernel fp_hip_summed_apply_weighted runs for these shapes, and its HBM bytes."""
+    p = EvPlan()
+    call("fp_hip_summed_apply_weighted_plan", ops.handle, dtype, dim, n_states, int(bool(accumulate)),
+         ctypes.byref(p))
     return p.as_dict()
 
 

@@ -220,9 +220,13 @@ fp_status fp_hip_op_expectation_plan(fp_hip_ops const *ops, int dtype, size_t di
  * writes; hoisted_groups and shared_groups are 0. */
 fp_status fp_hip_op_apply_plan(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_states, int accumulate,
                                fp_hip_ev_plan *plan);
-/* The same for fp_hip_summed_apply (the row-summed coefficients; no reference counterpart). */
+/* The same for fp_hip_summed_apply (the row-summed coefficients) and for
+ * fp_hip_summed_apply_weighted (per-state weights; the weight computation's bytes are not
+ * counted); no reference counterparts. */
 fp_status fp_hip_summed_apply_plan(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_states, int accumulate,
                                    fp_hip_ev_plan *plan);
+fp_status fp_hip_summed_apply_weighted_plan(fp_hip_ops const *ops, int dtype, size_t dim, size_t n_states,
+                                            int accumulate, fp_hip_ev_plan *plan);
 
 /* Per-string expectation values without coefficients, as a REAL (n_strings x n_states) matrix
  * ev(j,t) = <psi_t|P_j|psi_t> (FP64 for FP_C128, FP32 for FP_C64).  Pauli strings are Hermitian,

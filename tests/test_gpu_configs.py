@@ -102,9 +102,9 @@ def test_summed_ev_matrix_n14_vs_oracle_sample(summed14):
 @pytest.mark.parametrize("dtype", [C128, C64])
 @pytest.mark.parametrize("n,N,K,S", [(12, 200, 20, 16), (13, 97, 7, 5), (14, 60, 33, 32)])
 def test_summed_apply_and_weighted_bit_exact(hip, monkeypatch, kernel, dtype, n, N, K, S):
-    """SummedPauliOp::apply through the streaming kernel and (forced) the coset-tiled
-    kernel with the row-summed coefficients, and apply_weighted: IEEE-equal to the
-    oracle's sequential loops."""
+    """SummedPauliOp::apply (the row-summed coefficients) and apply_weighted (per-state
+    weights) through the streaming kernels and (forced) the coset-tiled kernel: IEEE-equal
+    to the oracle's sequential loops."""
     monkeypatch.setenv("FAST_PAULI_APPLY_KERNEL", kernel)
     H.say(f"summed apply / apply_weighted kernel={kernel} n={n} N={N} K={K} S={S} dtype={dtype}")
     cdt = H.CDT[dtype]
@@ -122,8 +122,9 @@ def test_summed_apply_and_weighted_bit_exact(hip, monkeypatch, kernel, dtype, n,
     d_data = _abi.DeviceBuffer.from_array(data)
     d_out = _abi.DeviceBuffer(psi.nbytes)
     c_dt = coeffs_nk.astype(cdt)
-    plan = _abi.summed_apply_plan(ops, dtype, dim, S)
-    assert plan["kernel"] == (_abi.FP_EV_TILED if kernel == "tiled" else _abi.FP_EV_STREAM)
+    want = _abi.FP_EV_TILED if kernel == "tiled" else _abi.FP_EV_STREAM
+    assert _abi.summed_apply_plan(ops, dtype, dim, S)["kernel"] == want
+    assert _abi.summed_apply_weighted_plan(ops, dtype, dim, S)["kernel"] == want
     _abi.call("fp_hip_summed_apply", ops.handle, dtype, d_in.ptr, d_out.ptr, dim, S, 0, None)
     got = d_out.to_array(cdt, psi.shape)
     exp = O.summed_apply(strings, c_dt, psi, np.zeros_like(psi), dtype=odt)
