@@ -85,10 +85,14 @@ TileGeom tile_geom(int dtype, size_t n_qubits, size_t n_states, bool expect)
     // (8 states): 2^9-row tiles, so groups of 9 strings (in the shared-leader form
     // for expectation values), against 8 strings with 256-byte rows; the slices of
     // a row are fetched by blocks of one XCD.  Apply gains the same way (256 random
-    // strings: 29 groups instead of 32).  With other state counts the slices would
-    // not align with rows.
+    // strings: 29 groups instead of 32), and also with any multiple of 8 states,
+    // whose slices stay 128-byte aligned (SummedPauliOp::apply at n = 14, 2000
+    // weight-3 strings, 1000 states: 149 groups instead of 172, 31.2 -> 28.2 ms;
+    // 64-byte slices: 123 groups but 36.0 ms).  With other state counts the slices
+    // would not align with rows.
     bool const pow2 = n_states != 0 && (n_states & (n_states - 1)) == 0;
-    uint32_t const row_target = env_u32("FAST_PAULI_TILE_ROW_BYTES", dtype == FP_C128 && pow2 ? 128u : 256u);
+    bool const slices128 = dtype == FP_C128 && (pow2 || (!expect && n_states % 8 == 0));
+    uint32_t const row_target = env_u32("FAST_PAULI_TILE_ROW_BYTES", slices128 ? 128u : 256u);
     // 128 KiB tiles (FAST_PAULI_TILE_KB=128) exist for the expectation kernel only; the
     // tiled apply kernel's tiles are 64 KiB
     uint32_t const tile_kb = expect && env_u32("FAST_PAULI_TILE_KB", 64u) >= 128u ? 128u : 64u;

@@ -224,14 +224,15 @@ def coset_entries():
     """Apply entries large enough (with a 1 KiB threshold) and with few enough string
     groups to run as coset items: power-of-two and other state counts, one and
     several state chunks, complex64 with one state per slot (odd counts) and two
-    (even counts), complex128, overwrite and accumulate; plus streaming apply and
+    (even counts), complex128 (128-byte row slices with 16 and 24 states), overwrite
+    and accumulate; plus streaming apply and
     expectation entries in the same 512-thread kernel."""
     A, E = _abi.FP_BATCH_APPLY, _abi.FP_BATCH_EXPECT
     specs = [
         (A, 12, 40, 3, C64, False), (A, 13, 64, 59, C64, False), (A, 12, 48, 64, C64, True),
         (A, 12, 30, 16, C128, False), (A, 12, 33, 5, C128, True), (A, 11, 24, 8, C64, False),
         (E, 10, 20, 7, C64, False), (A, 9, 12, 4, C64, False), (E, 12, 30, 16, C128, True),
-        (A, 12, 6, 1, C64, False), (A, 13, 40, 58, C64, False),
+        (A, 12, 6, 1, C64, False), (A, 13, 40, 58, C64, False), (A, 12, 40, 24, C128, False),
     ]
     return [Entry(*s, seed=40 + i) for i, s in enumerate(specs)]
 
@@ -257,7 +258,7 @@ def test_coset_items_bit_exact(hip, graph):
             b.launch()
             _abi.synchronize()
             info = b.info()
-            assert info["coset_entries"] == 7 and info["coset_items"] > 0, info
+            assert info["coset_entries"] == 8 and info["coset_items"] > 0, info
             for e, r in zip(entries, refs):
                 got = e.result()
                 if e.kind == _abi.FP_BATCH_APPLY:

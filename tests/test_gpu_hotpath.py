@@ -115,7 +115,7 @@ def test_op_apply_bit_exact(hip, dtype, n, N, S):
     ("random", 2, 5, 1), ("random", 8, 40, 3), ("random", 11, 64, 16), ("random", 13, 200, 7),
     ("random", 15, 50, 33), ("random", 16, 300, 2), ("weight2", 14, 300, 5), ("diagonal", 12, 40, 16),
     ("shared_x", 12, 64, 4), ("duplicates", 10, 33, 1), ("random", 13, 80, 6), ("random", 12, 64, 34),
-    ("random", 14, 100, 58),
+    ("random", 14, 100, 58), ("random", 13, 120, 24), ("random", 12, 64, 40),
 ])
 def test_op_apply_kernels(hip, monkeypatch, kernel, dtype, kind, n, N, S):
     """PauliOp::apply through the streaming kernel and the coset-tiled kernel
