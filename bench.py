@@ -81,6 +81,9 @@ def parse():
     ap.add_argument("--pmc-summed", action="store_true",
                     help="only run SummedPauliOp.expectation_value at n=14 (2000 strings, 200 operators, 1000 "
                          "states) once in complex128/FP64 and once in complex64/FP32, for profiler runs")
+    ap.add_argument("--pmc-summed-apply", action="store_true",
+                    help="only run SummedPauliOp.apply and apply_weighted at n=14 (2000 strings, 200 operators, "
+                         "1000 states, complex128) once each, for profiler runs")
     ap.add_argument("--pmc-batch", action="store_true",
                     help="only build the 256-operator uneven batch and launch it twice, for profiler runs")
     ap.add_argument("--pmc-gemm", action="store_true",
@@ -130,6 +133,42 @@ def pmc_summed(args) -> int:
     return 0
 
 
+def _summed_apply_operands(args, n, N, K, S):
+    """The SummedPauliOp.apply workload's operator set (N strings of weight <= 3, real h of
+    N x K) and apply_weighted data x (K x S): (ops, x, codes, h)."""
+    from fast_pauli import _abi
+    from fast_pauli import synthetic as syn
+
+    codes = syn.random_codes_max_weight(N, n, 3, seed=args.seed)
+    W = syn.random_real_matrix(K, N, seed=args.seed)
+    h = W.T.astype(np.complex128)
+    ops = _abi.Ops(codes, matrix=h)
+    x = syn.random_real_matrix(K, S, seed=args.seed + 1)
+    return ops, x, codes, h
+
+
+def pmc_summed_apply(args) -> int:
+    """SummedPauliOp.apply and apply_weighted on the n=14 operator set, complex128, once each."""
+    from fast_pauli import _abi
+    from fast_pauli import synthetic as syn
+
+    _abi.init(0)
+    n, N, K, S = 14, 2000, 200, 1000
+    dim = 1 << n
+    ops, x, _, _ = _summed_apply_operands(args, n, N, K, S)
+    d_in = _abi.DeviceBuffer(dim * S * 16)
+    _abi.call("fp_hip_fill_states", _abi.FP_C128, d_in.ptr, dim, S, syn.gpu_seed(args.seed), 1, None)
+    d_x = _abi.DeviceBuffer.from_array(np.ascontiguousarray(x, np.float64))
+    d_out = _abi.DeviceBuffer(dim * S * 16)
+    _abi.call("fp_hip_summed_apply", ops.handle, _abi.FP_C128, d_in.ptr, d_out.ptr, dim, S, 0, None)
+    _abi.call("fp_hip_summed_apply_weighted", ops.handle, _abi.FP_C128, d_in.ptr, d_x.ptr, d_out.ptr, dim, S, 0,
+              None)
+    _abi.synchronize()
+    log("pmc summed apply pass done: n=14, 2000 strings, 200 operators, 1000 states, complex128, "
+        f"{_abi.summed_apply_plan(ops, _abi.FP_C128, dim, S)['n_groups']} string groups")
+    return 0
+
+
 def pmc_batch(args) -> int:
     """The 256-operator uneven batch, default schedule: one build launch, one replay."""
     from fast_pauli import _abi, synthetic as syn
@@ -165,6 +204,8 @@ def main() -> int:
         return pmc_gemm()
     if args.pmc_summed:
         return pmc_summed(args)
+    if args.pmc_summed_apply:
+        return pmc_summed_apply(args)
     if args.pmc_batch:
         return pmc_batch(args)
     if args.gpus != 1:
@@ -577,11 +618,7 @@ def bench_summed_apply_n14(args, failures) -> dict:
     n, N, K, S = 14, 2000, 200, 1000
     dim = 1 << n
     log("workload: SummedPauliOp.apply / apply_weighted n=14, 2000 strings, 200 operators, 1000 states")
-    codes = syn.random_codes_max_weight(N, n, 3, seed=args.seed)
-    W = syn.random_real_matrix(K, N, seed=args.seed)
-    h = W.T.astype(np.complex128)
-    ops = _abi.Ops(codes, matrix=h)
-    x = syn.random_real_matrix(K, S, seed=args.seed + 1)
+    ops, x, codes, h = _summed_apply_operands(args, n, N, K, S)
     d_in = _abi.DeviceBuffer(dim * S * 16)
     _abi.call("fp_hip_fill_states", _abi.FP_C128, d_in.ptr, dim, S, syn.gpu_seed(args.seed), 1, None)
     d_x = _abi.DeviceBuffer.from_array(np.ascontiguousarray(x, np.float64))
