@@ -480,10 +480,16 @@ fp_status launch_op_apply_tiled(TileTables const *tt, TileGeom const &geo, int d
     uint32_t const chunks_y = uint32_t((n_states + (size_t(1) << geo.lsb) - 1) >> geo.lsb);
     if (chunks_y > 65535u)
         return fail(FP_ERR_INVALID_ARGUMENT, "too many states for the tiled apply kernel");
-    // about four tiles per block (a block with one tile has nothing to overlap its
-    // loads with), between 1024 and 8192 blocks; FAST_PAULI_TILE_BLOCKS overrides
+    // About four tiles per block (a block with one tile has nothing to overlap its loads
+    // with), between 1024 and 8192 blocks; but complex128 launches of at most 8192 tiles
+    // take one tile per block, which the hardware deals to CUs as they free up
+    // (SummedPauliOp::apply at n = 14, 1000 states, 4000 tiles per group launch: 28.2 ms
+    // with 1125 blocks of 3-4 tiles, 25.2 ms with 500 blocks of 8, 24.4 ms with one each;
+    // the complex64 group launches of a batch's 64-118 MiB entries, about 2048 tiles each,
+    // lose 2 ms with one tile per block).  FAST_PAULI_TILE_BLOCKS overrides.
     uint64_t const tiles = n_cosets * chunks_y;
-    uint64_t const target = env_u32("FAST_PAULI_TILE_BLOCKS", uint32_t(std::clamp<uint64_t>(tiles / 4, 1024, 8192)));
+    uint64_t const dflt = dtype == FP_C128 && tiles <= 8192 ? tiles : std::clamp<uint64_t>(tiles / 4, 1024, 8192);
+    uint64_t const target = env_u32("FAST_PAULI_TILE_BLOCKS", uint32_t(dflt));
     uint64_t const blocks_x = std::max<uint64_t>(1, std::min<uint64_t>(n_cosets, (target + chunks_y - 1) / chunks_y));
     uint32_t const sshift = (n_states & (n_states - 1)) == 0 ? log2u(n_states) : 64u;
     bool const wide = (uint64_t(dim) * n_states) >> 32 != 0 || env_u32("FAST_PAULI_TILE_WIDE", 0) == 1;
