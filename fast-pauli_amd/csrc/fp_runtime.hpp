@@ -65,8 +65,9 @@ inline size_t column_lds_bytes()
 // matrix kernels evaluate through a Walsh-Hadamard transform, or
 // FAST_PAULI_WHT_MIN (0 disables the transform).  The transform trades vector
 // and scalar instructions for LDS traffic: the complex128 coset kernel, whose
-// loop is LDS-bound, gains from the largest classes only (measured at n = 14,
-// 2000 weight-3 strings: 2.39 ms with 12, 2.32 ms with 40; complex64 1.26 / 1.35 ms).
+// loop is LDS-bound, gains from the larger classes only (32 strings and up; at n = 14,
+// 2000 weight-3 strings, 1000 states: 2.01 ms against 2.21 ms with 24); the complex64
+// LDS kernel gains from 12 (1.03 ms; 1.05 ms with 32, 1.31 ms with none).
 inline uint32_t wht_min_strings(uint32_t dflt = fpl::kWhtMin)
 {
     char const *v = std::getenv("FAST_PAULI_WHT_MIN");
