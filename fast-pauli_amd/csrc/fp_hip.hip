@@ -735,9 +735,7 @@ fp_status launch_ev_lds(fp_hip_ops const *ops, void const *states, void *ev, siz
                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(column_lds_bytes())));
         attr_set[which] = true;
     }
-    uint32_t gy = 1;
-    if (n_states < 1024)
-        gy = uint32_t(std::min<size_t>(ops->n_ev_chunks, (1024 + n_states - 1) / n_states));
+    uint32_t const gy = ev_matrix_split(n_states, 1, ops->n_ev_chunks);
     hipLaunchKernelGGL(fpl::k_ev_lds<T>, dim3(uint32_t(n_states), gy), dim3(fpl::kThreads), lds, st,
                        static_cast<V const *>(transposed), uint32_t(dim), ops->d_ev_chunks, ops->n_ev_chunks,
                        ops->n_ev_wht, ops->d_ev_strs, uint64_t(n_states), static_cast<T *>(ev));

@@ -629,9 +629,7 @@ fp_status launch_ev_coset(fp_hip_ops const *ops, TileTables *tt, CosetGeom const
     });
     if (attr != hipSuccess)
         return fail(FP_ERR_RUNTIME, "hipFuncSetAttribute failed: %s", hipGetErrorString(attr));
-    uint32_t gy = 1;
-    if (n_states < 1024)
-        gy = uint32_t(std::max<size_t>(1, std::min<size_t>(tt->max_chunks, (1024 + n_states - 1) / n_states)));
+    uint32_t const gy = ev_matrix_split(n_states, g.n_cos, tt->max_chunks);
     bool any_wht = false;
     auto const *hgroups = reinterpret_cast<fpt::TileGroup const *>(tt->h.data());
     for (uint32_t i = 0; i < tt->n_groups; ++i)
