@@ -139,12 +139,16 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
     // launch may need several (each block runs in one).  Modes 1 and 2 run 4
     // passes per step, complex64 in the paired lane form (2^(r+1) leader rows per
     // pass), which needs S_blk >= 2.
-    static_assert(MODE != 2 || sizeof(T) == 8, "the shared-leader instance is complex128 only");
     constexpr bool HOIST = MODE == 1;
     constexpr bool PAIRED = MODE >= 1 && sizeof(V) == 8;
     uint32_t const rrx = sizeof(V) == 8 ? r + 1u : r;
     bool const can_pair = sizeof(V) != 8 || lsb >= 1;
-    bool const g_shared = can_pair && ((grp.shared >> shared_bit(W, sizeof(V) == 8, rrx)) & 1u) != 0;
+    // complex64 takes the shared-leader form only with the paired 16-byte tile loads (full
+    // tiles, a power-of-two state count), which its instance assumes; the host applies the
+    // same rule when it picks the instances to launch
+    bool const pairs_ok = sizeof(V) != 8 || (sshift < 64u && lsb >= 1u && ((1u << d) << lsb) == kElems);
+    bool const g_shared =
+        can_pair && pairs_ok && ((grp.shared >> shared_bit(W, sizeof(V) == 8, rrx)) & 1u) != 0;
     bool const g_hoist = !g_shared && can_pair && ((grp.hoist >> hoist_bit(W, 4u, rrx)) & 1u) != 0;
     int const gmode = g_shared ? 2 : (g_hoist ? 1 : 0);
     if (gmode != MODE)
@@ -183,8 +187,12 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
     // elements 2 (tid + m * kThreads) and the one after (eoff[m]: its 16-byte index)
     typedef typename fpk::VecOf<float, 2>::type V4;
     constexpr uint32_t EPT2 = EPT / 2;
-    bool const vec2 = sizeof(V) == 8 && full && s_pow2 && lsb >= 1;
-    Idx eoff[EPT];
+    // the complex64 shared-leader instance runs only the paired form (see pairs_ok), so
+    // it keeps offsets for the 16-byte slots only
+    constexpr bool V2 = MODE == 2 && sizeof(V) == 8;
+    constexpr uint32_t NOFF = V2 ? EPT2 : EPT;
+    bool const vec2 = V2 || (sizeof(V) == 8 && full && s_pow2 && lsb >= 1);
+    Idx eoff[NOFF];
     if (vec2)
     {
 #pragma unroll
@@ -194,19 +202,23 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
             uint32_t const a = e >> lsb;
             uint64_t const roff = bt[a >> k] ^ uint64_t(a & lowmask);
             eoff[m] = Idx(((roff << sshift) | (s0 + (e & (sblk - 1u)))) >> 1);
-            eoff[EPT2 + m] = 0;
+            if constexpr (!V2)
+                eoff[EPT2 + m] = 0;
         }
     }
     else
     {
-#pragma unroll
-        for (uint32_t m = 0; m < EPT; ++m)
+        if constexpr (!V2)
         {
-            uint32_t const e = tid + m * kThreads;
-            uint32_t const a = e >> lsb;
-            bool const live = e < tile_elems && t_live;
-            uint64_t const roff = live ? (bt[a >> k] ^ uint64_t(a & lowmask)) : 0;
-            eoff[m] = Idx(s_pow2 ? ((roff << sshift) | (s0 + t)) : roff);
+#pragma unroll
+            for (uint32_t m = 0; m < EPT; ++m)
+            {
+                uint32_t const e = tid + m * kThreads;
+                uint32_t const a = e >> lsb;
+                bool const live = e < tile_elems && t_live;
+                uint64_t const roff = live ? (bt[a >> k] ^ uint64_t(a & lowmask)) : 0;
+                eoff[m] = Idx(s_pow2 ? ((roff << sshift) | (s0 + t)) : roff);
+            }
         }
     }
 
@@ -232,7 +244,9 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
                 return;
             }
         }
-        if (full)
+        if constexpr (V2)
+            return;
+        else if (full)
         {
 #pragma unroll
             for (uint32_t m = 0; m < EPT; ++m)
@@ -558,10 +572,13 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
         {
             if (vec2)
             {
+                // 16-byte slot tid + m * kThreads holds row (slot >> (lsb - 1)); MODE 2 stores
+                // row a at a ^ ((a >> 1) & 1) as below: row bit 1 is slot bit lsb
                 auto *tile4 = reinterpret_cast<V4 *>(smem);
+                uint32_t const tsw2 = MODE == 2 ? tid ^ (((tid >> lsb) & 1u) << (lsb - 1u)) : tid;
 #pragma unroll
                 for (uint32_t m = 0; m < EPT2; ++m)
-                    tile4[tid + m * kThreads] = V4{rg[2 * m][0], rg[2 * m][1], rg[2 * m + 1][0], rg[2 * m + 1][1]};
+                    tile4[tsw2 + m * kThreads] = V4{rg[2 * m][0], rg[2 * m][1], rg[2 * m + 1][0], rg[2 * m + 1][1]};
                 return;
             }
         }
@@ -592,24 +609,27 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
     // tile's loads are issued during the stores instead of after them
     // (complex128 only: complex64's paired 16-byte form keeps the separate passes)
     auto store_fetch = [&](V(&rg)[EPT], uint64_t cb) {
-        Idx const cbs = Idx(s_pow2 ? (cb << sshift) : cb);
-        auto index = [&](uint32_t m) {
-            return s_pow2 ? Idx(cbs ^ eoff[m]) : Idx((cbs ^ eoff[m]) * Idx(n_states) + Idx(s0 + t));
-        };
-        uint32_t const tsw = MODE == 2 ? tid ^ (((tid >> (lsb + 1u)) & 1u) << lsb) : tid;
-#pragma unroll
-        for (uint32_t m = 0; m < EPT; ++m)
+        if constexpr (sizeof(V) == 16)
         {
-            bool const live = full || tid + m * kThreads < tile_elems;
-            if (live)
-                tile[tsw + m * kThreads] = rg[m];
-            if (full)
-                rg[m] = __builtin_nontemporal_load(&states[index(m)]);
-            else
+            Idx const cbs = Idx(s_pow2 ? (cb << sshift) : cb);
+            auto index = [&](uint32_t m) {
+                return s_pow2 ? Idx(cbs ^ eoff[m]) : Idx((cbs ^ eoff[m]) * Idx(n_states) + Idx(s0 + t));
+            };
+            uint32_t const tsw = MODE == 2 ? tid ^ (((tid >> (lsb + 1u)) & 1u) << lsb) : tid;
+#pragma unroll
+            for (uint32_t m = 0; m < EPT; ++m)
             {
-                rg[m] = V(0);
-                if (live && t_live)
+                bool const live = full || tid + m * kThreads < tile_elems;
+                if (live)
+                    tile[tsw + m * kThreads] = rg[m];
+                if (full)
                     rg[m] = __builtin_nontemporal_load(&states[index(m)]);
+                else
+                {
+                    rg[m] = V(0);
+                    if (live && t_live)
+                        rg[m] = __builtin_nontemporal_load(&states[index(m)]);
+                }
             }
         }
     };

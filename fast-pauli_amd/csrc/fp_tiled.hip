@@ -228,13 +228,17 @@ fp_status choose_op_expect_tiled(fp_hip_ops const *ops, int dtype, size_t n_stat
 // leaders, 1 set up once, 0 general), by the kernel's own rule: modes 1 and 2
 // run 2^r leader rows per pass, r = 6 - lsb, or r + 1 for complex64 (two states
 // per lane, which needs lsb >= 1).
-int ev_group_mode(fpt::TileGroup const &g, int dtype, TileGeom const &geo)
+int ev_group_mode(fpt::TileGroup const &g, int dtype, TileGeom const &geo, size_t n_states)
 {
     bool const c64 = dtype != FP_C128;
     uint32_t const W = geo.tile_kb == 128 ? 16u : 8u;
     uint32_t const rrx = (c64 ? 7u : 6u) - geo.lsb;
     bool const can_pair = !c64 || geo.lsb >= 1;
-    if (can_pair && ((g.shared >> fpt::shared_bit(W, c64, rrx)) & 1u) != 0)
+    // complex64 shared leaders need the paired 16-byte tile loads: full tiles and a
+    // power-of-two state count (the kernel's own rule)
+    bool const pow2 = (n_states & (n_states - 1)) == 0;
+    bool const pairs_ok = !c64 || (pow2 && geo.lsb >= 1 && geo.d + geo.lsb == log2u(geo.tile_kb * 1024u / 8u));
+    if (can_pair && pairs_ok && ((g.shared >> fpt::shared_bit(W, c64, rrx)) & 1u) != 0)
         return 2;
     if (can_pair && ((g.hoist >> fpt::hoist_bit(W, 4u, rrx)) & 1u) != 0)
         return 1;
@@ -336,11 +340,8 @@ fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *stat
             set(reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<T, true, TKB, 0>), TKB << 10);
             set(reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<T, false, TKB, 1>), TKB << 10);
             set(reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<T, true, TKB, 1>), TKB << 10);
-            if constexpr (sizeof(T) == 8)
-            {
-                set(reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<T, false, TKB, 2>), TKB << 10);
-                set(reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<T, true, TKB, 2>), TKB << 10);
-            }
+            set(reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<T, false, TKB, 2>), TKB << 10);
+            set(reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<T, true, TKB, 2>), TKB << 10);
         };
         set4(double{}, std::integral_constant<int, 64>{});
         set4(float{}, std::integral_constant<int, 64>{});
@@ -359,22 +360,18 @@ fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *stat
     bool run_mode[3] = {false, false, false};
     auto const *hgroups = reinterpret_cast<fpt::TileGroup const *>(tt->h.data());
     for (uint32_t g = 0; g < tt->n_groups; ++g)
-        run_mode[ev_group_mode(hgroups[g], dtype, geo)] = true;
+        run_mode[ev_group_mode(hgroups[g], dtype, geo, n_states)] = true;
     auto launch = [&](auto T_tag, auto wide_tag, auto tkb_tag, void *typed_out_v) -> fp_status {
         typedef decltype(T_tag) T;
         typedef typename fpk::VecOf<T, 1>::type V;
         constexpr bool WIDE = decltype(wide_tag)::value;
         constexpr int TKB = decltype(tkb_tag)::value;
         auto const *typed_states = static_cast<V const *>(states);
-        // the shared-leader instance exists for complex128 only
-        constexpr int kTop = sizeof(T) == 8 ? 2 : 1;
         for (int mode = 2; mode >= 0; --mode)
         {
             if (!run_mode[mode])
                 continue;
-            if (mode > kTop)
-                return fail(FP_ERR_RUNTIME, "no shared-leader instance for this precision");
-            auto const kfn = mode == 2   ? &fpt::k_op_expect_tiled<T, WIDE, TKB, kTop>
+            auto const kfn = mode == 2   ? &fpt::k_op_expect_tiled<T, WIDE, TKB, 2>
                              : mode == 1 ? &fpt::k_op_expect_tiled<T, WIDE, TKB, 1>
                                          : &fpt::k_op_expect_tiled<T, WIDE, TKB, 0>;
             hipLaunchKernelGGL(kfn, grid, dim3(TKB * 8), lds, st, typed_states, uint64_t(n_states), sshift, geo.lsb,
@@ -705,7 +702,7 @@ extern "C" fp_status fp_hip_op_expectation_plan(fp_hip_ops const *ops, int dtype
     auto const *groups = reinterpret_cast<fpt::TileGroup const *>(tt->h.data());
     for (uint32_t g = 0; g < tt->n_groups; ++g)
     {
-        int const mode = fprt::ev_group_mode(groups[g], dtype, geo);
+        int const mode = fprt::ev_group_mode(groups[g], dtype, geo, n_states);
         plan->hoisted_groups += mode == 1 ? 1u : 0u;
         plan->shared_groups += mode == 2 ? 1u : 0u;
     }

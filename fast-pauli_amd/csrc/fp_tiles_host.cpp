@@ -105,8 +105,12 @@ TileGeom tile_geom(int dtype, size_t n_qubits, size_t n_states, bool expect)
     g.lsb = log2u(sblk);
     g.d = std::min<uint32_t>(log2u(elems / sblk), uint32_t(n_qubits));
     g.k = 0;
-    if (sblk >= n_states) // one block covers every state: 2^k consecutive rows are contiguous
-        while ((uint64_t(1) << g.k) * n_states * esz < 256 && g.k < g.d)
+    // one block covers every state: 2^k consecutive rows are contiguous, at least 256
+    // bytes; complex64 expectation tiles take 128 bytes like complex128's row slices
+    // (S = 16: one fewer low row bit, so groups of 9 strings instead of 8)
+    uint32_t const seg = expect && dtype != FP_C128 ? 128u : 256u;
+    if (sblk >= n_states)
+        while ((uint64_t(1) << g.k) * n_states * esz < seg && g.k < g.d)
             ++g.k;
     // FAST_PAULI_TILE_K: at least this many low row bits in every tile (longer contiguous segments)
     uint32_t const kmin = std::min<uint32_t>(env_u32("FAST_PAULI_TILE_K", 0), g.d);
@@ -346,12 +350,14 @@ bool shared_basis(HostGroup &g, StringTable const *ops, uint32_t k, uint32_t dh)
 // time in the set-up-once form (the shared form measured no faster there).
 uint32_t shared_bits(uint32_t d, uint32_t k, size_t n_str)
 {
+    // the same rule for complex128 (2^rr leader rows per pass, one state per lane) and
+    // complex64 (the paired lanes: two states per lane, 2^rr = 2^(7 - lsb) rows per pass)
     uint32_t bits = 0;
     for (uint32_t W : {8u, 16u})
         for (uint32_t rr = 0; rr <= 6; ++rr)
             if (k < rr && d >= rr + 1 && (uint64_t(1) << (d - 1 - rr)) == 4u * W &&
                 n_str <= size_t(fpt::max_shared(W)))
-                bits |= 1u << fpt::shared_bit(W, false, rr);
+                bits |= (1u << fpt::shared_bit(W, false, rr)) | (1u << fpt::shared_bit(W, true, rr));
     return bits;
 }
 

@@ -276,7 +276,8 @@ def test_op_expectation_kernels(hip, monkeypatch, kernel, dtype, kind, n, N, S):
 def test_op_expectation_plan(hip, monkeypatch):
     """fp_hip_op_expectation_plan: at 16 complex128 states, tiles take 128-byte
     row slices (8 states, 2^9-row tiles) and random strings group about 9 per
-    tile; complex64 takes 16 states per block (2^9 rows, 8 strings per group);
+    tile; complex64 takes 16 states per block (2^9 rows, 128-byte row segments, about 9
+    strings per group, shared leaders with a power-of-two state count);
     the bytes follow the groups; a forced streaming kernel reports one state
     read per string."""
     H.say("op_expectation plan")
@@ -289,8 +290,13 @@ def test_op_expectation_plan(hip, monkeypatch):
     assert p["hbm_bytes"] == p["n_groups"] * (1 << n) * S * 16
     assert N // 18 <= p["shared_groups"] and p["shared_groups"] + p["hoisted_groups"] <= p["n_groups"]
     p64 = _abi.op_expectation_plan(ops, C64, 1 << n, S)
-    assert p64["tile_rows_log2"] == 9 and p64["states_per_block"] == 16 and N // 8 <= p64["n_groups"] <= N // 4
-    assert p64["shared_groups"] == 0 and 0 < p64["hoisted_groups"] <= p64["n_groups"]
+    assert p64["tile_rows_log2"] == 9 and p64["states_per_block"] == 16 and N // 9 <= p64["n_groups"] <= N // 4
+    assert p64["hbm_bytes"] == p64["n_groups"] * (1 << n) * S * 8
+    assert N // 18 <= p64["shared_groups"] and p64["shared_groups"] + p64["hoisted_groups"] <= p64["n_groups"]
+    # complex64 shared leaders need the paired loads: a non-power-of-two state count
+    # runs the other instances
+    p64b = _abi.op_expectation_plan(ops, C64, 1 << n, 12)
+    assert p64b["shared_groups"] == 0
     single = _abi.Ops(syn.random_codes(1, n, seed=5))
     assert _abi.op_expectation_plan(single, C128, 1 << n, S)["kernel"] == _abi.FP_EV_STREAM
     monkeypatch.setenv("FAST_PAULI_EV_KERNEL", "stream")
@@ -303,9 +309,10 @@ def test_op_expectation_plan(hip, monkeypatch):
 @pytest.mark.parametrize("dtype", [C128, C64])
 @pytest.mark.parametrize("S", [1, 2, 16, 24])
 def test_tiled_expectation_both_instances(hip, monkeypatch, dtype, S):
-    """The tiled kernel's instances: complex128 groups whose strings all share
-    leader rows (at most 8 strings, rebased so that every local x mask has the
-    lowest span bit) run in the shared-leader instance, other groups whose waves
+    """The tiled kernel's instances: groups whose strings all share leader rows (at
+    most 9 strings, rebased so that every local x mask has the lowest span bit) run in
+    the shared-leader instance (complex64: with a power-of-two state count and full
+    tiles, two states per lane), other groups whose waves
     each own one single-string unit in the set-up-once instance (two states per
     lane for complex64, which needs 2 states or more per block), the rest in the
     per-tile instance; launches mixing them equal the oracle, and so do launches
@@ -330,9 +337,9 @@ def test_tiled_expectation_both_instances(hip, monkeypatch, dtype, S):
         # at most 8 chunks of one string each
         if dtype == C64 and S == 1:
             assert h == 0, name  # one state per block: no two-state lanes
-        elif name == "random" and dtype == C128:
+        elif name == "random" and (dtype == C128 or S & (S - 1) == 0):
             assert p["shared_groups"] > 0, (name, p)
-        elif name == "random":  # the shared-leader instance is complex128 only
+        elif name == "random":  # complex64 shared leaders need a power-of-two state count
             assert p["shared_groups"] == 0 and p["hoisted_groups"] > 0, (name, p)
         elif name == "shared_x":
             assert h == 0, (name, p)  # 40 strings of 4 x masks: one group, chunks of 4
