@@ -609,7 +609,23 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
     // tile's loads are issued during the stores instead of after them
     // (complex128 only: complex64's paired 16-byte form keeps the separate passes)
     auto store_fetch = [&](V(&rg)[EPT], uint64_t cb) {
-        if constexpr (sizeof(V) == 16)
+        if constexpr (V2)
+        {
+            // the complex64 shared-leader instance: 16-byte slots (two states), rows swizzled
+            auto *tile4 = reinterpret_cast<V4 *>(smem);
+            auto const *st4 = reinterpret_cast<V4 const *>(states);
+            Idx const cbs4 = Idx(cb << (sshift - 1u));
+            uint32_t const tsw2 = tid ^ (((tid >> lsb) & 1u) << (lsb - 1u));
+#pragma unroll
+            for (uint32_t m = 0; m < EPT2; ++m)
+            {
+                tile4[tsw2 + m * kThreads] = V4{rg[2 * m][0], rg[2 * m][1], rg[2 * m + 1][0], rg[2 * m + 1][1]};
+                V4 const v = __builtin_nontemporal_load(&st4[cbs4 ^ eoff[m]]);
+                rg[2 * m] = V{v[0], v[1]};
+                rg[2 * m + 1] = V{v[2], v[3]};
+            }
+        }
+        else if constexpr (sizeof(V) == 16)
         {
             Idx const cbs = Idx(s_pow2 ? (cb << sshift) : cb);
             auto index = [&](uint32_t m) {
@@ -648,8 +664,9 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
             uint64_t const cbn = next_base(cb);
             bool const more = q + q_step < n_cosets;
             // complex128: the next tile's loads go out during the stores (n = 24, 16 states,
-            // 256 strings: 19.10 -> 18.64 ms); complex64 loses with it (10.24 -> 10.62 ms)
-            if constexpr (sizeof(V) == 16)
+            // 256 strings: 19.10 -> 18.64 ms); complex64 lost with it in the set-up-once
+            // instance (10.24 -> 10.62 ms)
+            if constexpr (sizeof(V) == 16 || V2)
             {
                 if (more)
                     store_fetch(reg, cbn);
