@@ -243,6 +243,7 @@ def _structured_codes(kind, N, n, seed):
     ("random", 2, 5, 1), ("random", 8, 40, 3), ("random", 11, 64, 16), ("random", 13, 200, 7),
     ("random", 15, 50, 33), ("random", 16, 300, 2), ("random", 18, 64, 16), ("weight2", 14, 300, 5),
     ("diagonal", 12, 40, 16), ("shared_x", 12, 64, 4), ("duplicates", 10, 33, 1), ("random", 20, 24, 1),
+    ("random", 8, 40, 16), ("random", 9, 40, 32), ("random", 14, 120, 64), ("random", 12, 90, 4),
 ])
 def test_op_expectation_kernels(hip, monkeypatch, kernel, dtype, kind, n, N, S):
     """PauliOp::expectation_value through the streaming kernel, the coset-tiled
