@@ -96,7 +96,8 @@ extern "C" fp_status fp_hip_summed_square(fp_hip_ops const *ops, int dtype, void
     size_t const o_binom = 0, o_first = round256(64 * cols * 8), o_pow3 = o_first + round256(cols * 8);
     size_t const o_kin = o_pow3 + round256(cols * 8), o_kout = o_kin + round256(n_pairs * 4);
     size_t const o_vin = o_kout + round256(n_pairs * 4), o_vout = o_vin + round256(n_pairs * 8);
-    size_t const o_starts = o_vout + round256(n_pairs * 8), o_cub = o_starts + round256((A + 1) * 8);
+    size_t const o_starts = o_vout + round256(n_pairs * 8), o_recs = o_starts + round256((A + 1) * 8);
+    size_t const o_cub = o_recs + round256(A * 16);
     void *scratch = nullptr;
     fp_status s = scratch_get(st, o_cub + cub_bytes, &scratch);
     if (s != FP_OK)
@@ -110,6 +111,7 @@ extern "C" fp_status fp_hip_summed_square(fp_hip_ops const *ops, int dtype, void
     auto *v_in = reinterpret_cast<uint64_t *>(base + o_vin);
     auto *v_out = reinterpret_cast<uint64_t *>(base + o_vout);
     auto *starts = reinterpret_cast<uint64_t *>(base + o_starts);
+    auto *recs = reinterpret_cast<ulonglong2 *>(base + o_recs);
     hipLaunchKernelGGL(k_rank_tables, dim3(1), dim3(64), 0, st, n, W, binom, first, pow3);
     FP_LAUNCHED();
     fpalg::RankTables const tab{W, binom, first, pow3};
@@ -121,11 +123,16 @@ extern "C" fp_status fp_hip_summed_square(fp_hip_ops const *ops, int dtype, void
     hipLaunchKernelGGL(fpalg::k_segment_starts, dim3(uint32_t((A + 1 + 255) / 256)), dim3(256), 0, st, k_out,
                        n_pairs, A, starts);
     FP_LAUNCHED();
+    hipLaunchKernelGGL(fpalg::k_segment_records, dim3(uint32_t((A + 255) / 256)), dim3(256), 0, st, starts, v_out, A,
+                       recs);
+    FP_LAUNCHED();
     uint64_t const n_work = A * K;
     if (n_work / 256 >= uint64_t(0x7fffffff))
         return fail(FP_ERR_UNSUPPORTED, "the square's coefficient matrix is too large for one launch");
-    // a resident grid (FAST_PAULI_SQUARE_BLOCKS_PER_CU blocks per CU, default 32: the
-    // contraction is bound by the latency of its dependent loads, so more waves help)
+    // a grid of FAST_PAULI_SQUARE_BLOCKS_PER_CU blocks per CU (default 64) whose threads
+    // stride over the outputs: the contraction is bound by the latency of its dependent
+    // loads, so more waves help (n = 14, 2000 strings, 200 operators: 8 blocks per CU
+    // 6.92 ms, 16 6.45, 32 6.16, 64 5.95, 128 5.93)
     static int const cus = [] {
         int dev = 0, n = 256;
         hipDeviceProp_t prop;
@@ -134,7 +141,7 @@ extern "C" fp_status fp_hip_summed_square(fp_hip_ops const *ops, int dtype, void
         return n;
     }();
     char const *bpc_env = std::getenv("FAST_PAULI_SQUARE_BLOCKS_PER_CU");
-    uint64_t const bpc = bpc_env && std::atoi(bpc_env) > 0 ? uint64_t(std::atoi(bpc_env)) : 32;
+    uint64_t const bpc = bpc_env && std::atoi(bpc_env) > 0 ? uint64_t(std::atoi(bpc_env)) : 64;
     dim3 const grid(uint32_t(std::min<uint64_t>((n_work + 255) / 256, uint64_t(cus) * bpc)));
     bool const small = n_work + uint64_t(grid.x) * 256 <= uint64_t(0xffffffffu);
     typedef fpk::VecOf<double, 1>::type V2;
@@ -144,12 +151,12 @@ extern "C" fp_status fp_hip_summed_square(fp_hip_ops const *ops, int dtype, void
     if (dtype == FP_C128)
     {
         auto const kfn = small ? &fpalg::k_square_contract<double, true> : &fpalg::k_square_contract<double, false>;
-        hipLaunchKernelGGL(kfn, grid, dim3(256), 0, st, v_out, starts, h128, K, A, static_cast<V2 *>(out));
+        hipLaunchKernelGGL(kfn, grid, dim3(256), 0, st, v_out, recs, h128, K, A, static_cast<V2 *>(out));
     }
     else
     {
         auto const kfn = small ? &fpalg::k_square_contract<float, true> : &fpalg::k_square_contract<float, false>;
-        hipLaunchKernelGGL(kfn, grid, dim3(256), 0, st, v_out, starts, h64, K, A, static_cast<F2 *>(out));
+        hipLaunchKernelGGL(kfn, grid, dim3(256), 0, st, v_out, recs, h64, K, A, static_cast<F2 *>(out));
     }
     FP_LAUNCHED();
     return FP_OK;

@@ -105,14 +105,29 @@ __global__ __launch_bounds__(256) void k_segment_starts(uint32_t const *__restri
     starts[a] = lo;
 }
 
+// recs[a] = (start | count << 32, first value) of the segment of a, so that the
+// contraction reaches its first pair with one load (n_keys < 2^31).
+__global__ __launch_bounds__(256) void k_segment_records(uint64_t const *__restrict__ starts,
+                                                         uint64_t const *__restrict__ vals, uint64_t n_out,
+                                                         ulonglong2 *__restrict__ recs)
+{
+    uint64_t const a = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (a >= n_out)
+        return;
+    uint64_t const s = starts[a], e = starts[a + 1];
+    recs[a] = make_ulonglong2(s | ((e - s) << 32), e > s ? vals[s] : 0ull);
+}
+
 // out(a, k) = sum over the segment of a of (i^e * h(i,k)) * h(j,k), in sorted order.
 // Threads stride over the (a, k) outputs, k fastest (a resident grid: the outputs
 // are many and short, so the waves stay resident instead of being launched one
 // per 64 outputs); SMALL: every index < 2^32 (32-bit division).  Each output sums
 // its own segment in sorted order.
+// A thread's chain of dependent loads is its segment record, then the two h values of
+// each pair (later pairs' records are independent of the first pair's h loads).
 template <class T, bool SMALL>
 __global__ __launch_bounds__(256) void k_square_contract(uint64_t const *__restrict__ vals,
-                                                         uint64_t const *__restrict__ starts,
+                                                         ulonglong2 const *__restrict__ recs,
                                                          typename fpk::VecOf<T, 1>::type const *__restrict__ h,
                                                          uint64_t n_ops, uint64_t n_out,
                                                          typename fpk::VecOf<T, 1>::type *__restrict__ out)
@@ -135,9 +150,11 @@ __global__ __launch_bounds__(256) void k_square_contract(uint64_t const *__restr
             k = idx - a * n_ops;
         }
         T ar = T(0), ai = T(0);
-        for (uint64_t s = starts[a], e = starts[a + 1]; s < e; ++s)
+        ulonglong2 const rec = recs[a];
+        uint64_t const s0 = rec.x & 0xffffffffull, cnt = rec.x >> 32;
+        for (uint64_t c = 0; c < cnt; ++c)
         {
-            uint64_t const v = vals[s];
+            uint64_t const v = c == 0 ? rec.y : vals[s0 + c];
             uint32_t const ph = uint32_t(v) & 3u;
             uint64_t const i = v >> 33, j = (v >> 2) & 0x7fffffffu;
             // i^ph as a complex number with exact components
@@ -153,7 +170,7 @@ __global__ __launch_bounds__(256) void k_square_contract(uint64_t const *__restr
         V r;
         r[0] = ar;
         r[1] = ai;
-        out[idx] = r;
+        __builtin_nontemporal_store(r, &out[idx]);
     }
 }
 
