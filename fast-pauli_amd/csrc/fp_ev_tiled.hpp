@@ -606,8 +606,9 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
     };
     // store_tile(rg) followed by fetch(rg, cb), element by element: each register is
     // reloaded for the next tile as soon as it has been written to LDS, so the next
-    // tile's loads are issued during the stores instead of after them
-    // (complex128 only: complex64's paired 16-byte form keeps the separate passes)
+    // tile's loads are issued during the stores instead of after them (complex128, and
+    // the complex64 shared-leader instance; the other complex64 instances keep the
+    // separate passes)
     auto store_fetch = [&](V(&rg)[EPT], uint64_t cb) {
         if constexpr (V2)
         {

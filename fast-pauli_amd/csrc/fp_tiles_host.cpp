@@ -345,9 +345,9 @@ bool shared_basis(HostGroup &g, StringTable const *ops, uint32_t k, uint32_t dh)
 // TileGroup::shared of a shared-leader group of dimension d: the shared form
 // needs full 64 or 128 KiB tiles (2^(d-1) leader rows = 4 passes per wave of
 // 2^rr rows), the leader bit k inside the lane part (k < rr) and at most
-// max_shared(W) strings (the instance's per-string registers).  It is used
-// for complex128 only: complex64 tiles already evaluate at about their load
-// time in the set-up-once form (the shared form measured no faster there).
+// max_shared(W) strings (the instance's per-string registers).  Complex64 also
+// needs full tiles and a power-of-two state count (the paired 16-byte loads),
+// which the launch checks.
 uint32_t shared_bits(uint32_t d, uint32_t k, size_t n_str)
 {
     // the same rule for complex128 (2^rr leader rows per pass, one state per lane) and
