@@ -494,10 +494,40 @@ void check_errors()
               "dh = 0 accepted with x bits above k");
 }
 
+// The tile geometries the kernels are tuned for: complex128 takes 128-byte row slices
+// (8 states) for power-of-two state counts and, for apply, any multiple of 8; complex64
+// expectation tiles keep 128-byte runs of consecutive rows (k = 0 at 16 states), the
+// other tiles 256-byte runs.
+void check_geometry_rules()
+{
+    struct G
+    {
+        int dtype;
+        size_t n, S;
+        bool expect;
+        uint32_t lsb, d, k;
+    };
+    G const cases[] = {
+        {FP_C128, 24, 16, true, 3, 9, 0},    {FP_C128, 24, 16, false, 3, 9, 0},
+        {FP_C128, 14, 1000, false, 3, 9, 0}, {FP_C128, 14, 1000, true, 4, 8, 0},
+        {FP_C128, 14, 24, false, 3, 9, 0},   {FP_C128, 14, 24, true, 4, 8, 0},
+        {FP_C64, 24, 16, true, 4, 9, 0},     {FP_C64, 24, 16, false, 4, 9, 1},
+        {FP_C64, 24, 1, true, 0, 13, 4},     {FP_C64, 24, 1, false, 0, 13, 5},
+    };
+    for (G const &g : cases)
+    {
+        fprt::TileGeom const t = fprt::tile_geom(g.dtype, g.n, g.S, g.expect);
+        CHECK(t.lsb == g.lsb && t.d == g.d && t.k == g.k && t.dh == t.d - t.k,
+              "tile_geom(dtype %d, n %zu, S %zu, expect %d): lsb %u d %u k %u, want %u %u %u", g.dtype, g.n, g.S,
+              int(g.expect), t.lsb, t.d, t.k, g.lsb, g.d, g.k);
+    }
+}
+
 } // namespace
 
 int main()
 {
+    check_geometry_rules();
     Rng r{2024};
     struct Case
     {
@@ -512,7 +542,7 @@ int main()
         Strings const S = random_strings(c.n, c.N, c.kind, r);
         // the kernels' geometries: apply and expectation tiles for several state counts
         for (int dtype : {FP_C128, FP_C64})
-            for (size_t n_states : {size_t(1), size_t(3), size_t(16), size_t(59), size_t(64)})
+            for (size_t n_states : {size_t(1), size_t(3), size_t(16), size_t(24), size_t(59), size_t(64)})
             {
                 fprt::TileGeom const ga = fprt::tile_geom(dtype, c.n, n_states, false);
                 if (ga.dh > 0 && ga.k + ga.dh <= c.n)
