@@ -342,8 +342,11 @@ __device__ __forceinline__ void chunk_rows_k(typename fpk::VecOf<T, 1>::type con
 {
     switch (ch.count)
     {
-    case 1: chunk_rows_n<T, U, 1, KIND>(col, lane, ch, n_blocks64, zhi, acc); break;
-    case 2: chunk_rows_n<T, U, 2, KIND>(col, lane, ch, n_blocks64, zhi, acc); break;
+    // one- and two-string chunks (most of them for weight-limited sets) keep 8 row
+    // blocks of loads in flight per step instead of 4 (n = 14, 2000 weight-3 strings,
+    // 1000 states: complex64 1.007 -> 0.986 ms, complex128 unchanged)
+    case 1: chunk_rows_n<T, 2 * U, 1, KIND>(col, lane, ch, n_blocks64, zhi, acc); break;
+    case 2: chunk_rows_n<T, 2 * U, 2, KIND>(col, lane, ch, n_blocks64, zhi, acc); break;
     case 3: chunk_rows_n<T, U, 3, KIND>(col, lane, ch, n_blocks64, zhi, acc); break;
     case 4: chunk_rows_n<T, U, 4, KIND>(col, lane, ch, n_blocks64, zhi, acc); break;
     case 5: chunk_rows_n<T, U, 5, KIND>(col, lane, ch, n_blocks64, zhi, acc); break;
