@@ -570,10 +570,11 @@ def bench_summed_n14(args) -> dict:
              ("c64_bf16x3_mfma", _abi.FP_C64, _abi.FP_CONTRACT_BF16X3, d64, out64)]
     ref = None
     for name, dt, mode, src, dst in modes:
+        # 20 timed calls: over 3 the first calls' clock ramp adds about 0.15 ms to a 2 ms call
         ms = _time_ms(lambda: _abi.call("fp_hip_summed_expectation", ops.handle, dt, mode, src.ptr, dst.ptr, dim, S,
-                                        0, None), 3)
+                                        0, None), 20)
         ms1 = _time_ms(lambda: _abi.call("fp_hip_strings_expectation_matrix", ops.handle, dt, src.ptr, ev.ptr, dim,
-                                         S, None), 3)
+                                         S, None), 20)
         got = dst.to_array(np.complex128 if dt == _abi.FP_C128 else np.complex64, (K, S)).astype(np.complex128)
         entry = {"ms": ms, "stage1_ms": ms1, "strings_x_states_per_s": N * S / (ms / 1e3)}
         if ref is None:
