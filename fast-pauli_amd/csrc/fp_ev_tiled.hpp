@@ -92,7 +92,10 @@ __device__ __forceinline__ uint64_t coset_base(uint64_t q, uint32_t k, TileGroup
 // owns a single one-string unit keeps its addresses, signs and scale in
 // registers for all of the block's cosets, and coset bases advance by one
 // deposited addition per tile (see below) instead of a pivot-insertion loop.
-template <class T, bool WIDE, int TKB, int MODE>
+// MODE 3 is MODE 2 with each string's partner offset and coset sign mask packed in one
+// 32-bit word (fewer scalar registers, fewer spills in the string loop); the host takes it
+// when every coset index and every byte offset in the tile is below 2^16.
+template <class T, bool WIDE, int TKB, int MODE_>
 __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typename fpk::VecOf<T, 1>::type const *__restrict__ states,
                                                               uint64_t n_states, uint32_t sshift, uint32_t lsb,
                                                               uint32_t k, uint32_t d, uint32_t ranges,
@@ -105,6 +108,9 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
                                                               ShStr const *__restrict__ sh,
                                                               fpk::Dbl2 *__restrict__ partial)
 {
+    constexpr int MODE = MODE_ == 3 ? 2 : MODE_; // the evaluation form
+    constexpr bool PK = MODE_ == 3;
+    static_assert(!PK || TKB * 1024 <= 65536, "packed offsets: tile offsets below 2^16");
     typedef typename fpk::VecOf<T, 1>::type V;
     typedef typename fpk::RowAcc<T>::type Acc;
     constexpr int kThreads = TileCfg<TKB>::kThreads;
@@ -768,6 +774,9 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
         // per string: case (Im flag * 4 + ci) in 3 bits of `cases`, zi in 2 bits of `zis`
         uint32_t cases = 0, zis = 0;
         uint32_t zcs[KMS]; // coset sign masks (coset indices < 2^32: host condition)
+        // PK: partner offset in bits 0-15, coset sign mask in bits 16-31 (coset indices and
+        // tile byte offsets below 2^16: host condition)
+        uint32_t pz[KMS];
         Acc acc_s[KMS][NS];
 #pragma unroll
         for (int s = 0; s < KMS; ++s)
@@ -786,22 +795,35 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
                 cases |= (((ss[s].imag ? 4u : 0u) + ((A >> (rr + 1u)) & 3u)) << (3 * s));
                 zis |= ((Z >> (rr + 1u)) & 3u) << (2 * s);
             }
+            pz[s] = 0;
+            if constexpr (PK)
+                pz[s] = __builtin_amdgcn_readfirstlane(px[s] | (zcs[s] << 16));
         }
         cases = __builtin_amdgcn_readfirstlane(cases);
         zis = __builtin_amdgcn_readfirstlane(zis);
         auto compute = [&](uint64_t q) {
             // coset signs of every string
             uint32_t cneg = 0;
+            if constexpr (PK)
+            {
+                uint32_t const qq = uint32_t(q) << 16; // low half clear: masks out the offsets
 #pragma unroll
-            for (int s = 0; s < KMS; ++s)
-                cneg |= uint32_t(__popc(uint32_t(q) & zcs[s]) & 1) << s;
+                for (int s = 0; s < KMS; ++s)
+                    cneg |= uint32_t(__popc(qq & pz[s]) & 1) << s;
+            }
+            else
+            {
+#pragma unroll
+                for (int s = 0; s < KMS; ++s)
+                    cneg |= uint32_t(__popc(uint32_t(q) & zcs[s]) & 1) << s;
+            }
             LV L[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i)
                 L[i] = *reinterpret_cast<LV const *>(smem + (la0 + uint32_t(i) * STRIDE));
             // partner values of string s
             auto load_R = [&](int s, LV(&R)[4]) {
-                uint32_t const base = la0 ^ px[s];
+                uint32_t const base = la0 ^ (PK ? (pz[s] & 0xffffu) : px[s]);
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
                     R[j] = *reinterpret_cast<LV const *>(smem + (base + uint32_t(j) * STRIDE));

@@ -345,6 +345,8 @@ fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *stat
         };
         set4(double{}, std::integral_constant<int, 64>{});
         set4(float{}, std::integral_constant<int, 64>{});
+        set(reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<double, false, 64, 3>), 64 << 10);
+        set(reinterpret_cast<void const *>(&fpt::k_op_expect_tiled<float, false, 64, 3>), 64 << 10);
         set4(double{}, std::integral_constant<int, 128>{});
         set4(float{}, std::integral_constant<int, 128>{});
     });
@@ -367,13 +369,20 @@ fp_status run_op_expect_tiled(fp_hip_ops const *ops, int dtype, void const *stat
         constexpr bool WIDE = decltype(wide_tag)::value;
         constexpr int TKB = decltype(tkb_tag)::value;
         auto const *typed_states = static_cast<V const *>(states);
+        // shared leaders with each string's partner offset and coset sign mask in one word
+        // when every coset index is below 2^16 (64 KiB tiles: offsets below 2^16 too);
+        // FAST_PAULI_EV_PACK=2 keeps them apart
+        bool const pack = n_cosets <= 65536u && env_u32("FAST_PAULI_EV_PACK", 1) == 1;
         for (int mode = 2; mode >= 0; --mode)
         {
             if (!run_mode[mode])
                 continue;
-            auto const kfn = mode == 2   ? &fpt::k_op_expect_tiled<T, WIDE, TKB, 2>
-                             : mode == 1 ? &fpt::k_op_expect_tiled<T, WIDE, TKB, 1>
-                                         : &fpt::k_op_expect_tiled<T, WIDE, TKB, 0>;
+            auto kfn = mode == 2   ? &fpt::k_op_expect_tiled<T, WIDE, TKB, 2>
+                       : mode == 1 ? &fpt::k_op_expect_tiled<T, WIDE, TKB, 1>
+                                   : &fpt::k_op_expect_tiled<T, WIDE, TKB, 0>;
+            if constexpr (!WIDE && TKB == 64)
+                if (mode == 2 && pack)
+                    kfn = &fpt::k_op_expect_tiled<T, false, 64, 3>;
             hipLaunchKernelGGL(kfn, grid, dim3(TKB * 8), lds, st, typed_states, uint64_t(n_states), sshift, geo.lsb,
                                geo.k, geo.d, ranges, uint32_t(n_parts), chunks_y, n_cosets, d_groups, d_btab,
                                static_cast<fpt::TileChunk const *>(d_chunks),

@@ -307,9 +307,10 @@ def test_op_expectation_plan(hip, monkeypatch):
         _abi.op_expectation_plan(ops, C128, 1 << (n - 1), S)
 
 
+@pytest.mark.parametrize("pack", ["1", "2"])
 @pytest.mark.parametrize("dtype", [C128, C64])
 @pytest.mark.parametrize("S", [1, 2, 16, 24])
-def test_tiled_expectation_both_instances(hip, monkeypatch, dtype, S):
+def test_tiled_expectation_both_instances(hip, monkeypatch, dtype, S, pack):
     """The tiled kernel's instances: groups whose strings all share leader rows (at
     most 9 strings, rebased so that every local x mask has the lowest span bit) run in
     the shared-leader instance (complex64: with a power-of-two state count and full
@@ -317,9 +318,11 @@ def test_tiled_expectation_both_instances(hip, monkeypatch, dtype, S):
     each own one single-string unit in the set-up-once instance (two states per
     lane for complex64, which needs 2 states or more per block), the rest in the
     per-tile instance; launches mixing them equal the oracle, and so do launches
-    of one kind alone."""
-    H.say(f"tiled expectation instances dtype={dtype} S={S}")
+    of one kind alone.  pack 1: the shared-leader instance with each string's partner
+    offset and coset sign mask in one word (the default below 2^16 cosets); 2: apart."""
+    H.say(f"tiled expectation instances dtype={dtype} S={S} pack={pack}")
     monkeypatch.setenv("FAST_PAULI_EV_KERNEL", "tiled")
+    monkeypatch.setenv("FAST_PAULI_EV_PACK", pack)
     n = 16
     cdt = H.CDT[dtype]
     psi = syn.random_states(1 << n, S, seed=41).astype(cdt)
@@ -352,6 +355,30 @@ def test_tiled_expectation_both_instances(hip, monkeypatch, dtype, S):
         tol = 1e-12 if dtype == C128 else 1e-5
         assert H.ev_scaled_err(got, exp, coeffs, psi) <= tol, name
         assert np.array_equal(H.op_expectation(ops, psi, dtype), got), name
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
+@pytest.mark.parametrize("n,S", [(14, 16), (18, 8), (20, 8)])
+def test_tiled_expectation_packed_equals_unpacked(hip, monkeypatch, dtype, n, S):
+    """The shared-leader instance with packed per-string words gives the same bits as
+    the unpacked one (same arithmetic in the same order), and both match the oracle."""
+    H.say(f"tiled expectation packed vs unpacked dtype={dtype} n={n} S={S}")
+    monkeypatch.setenv("FAST_PAULI_EV_KERNEL", "tiled")
+    cdt = H.CDT[dtype]
+    codes = syn.random_codes(32, n, seed=n + S)
+    coeffs = syn.random_coeffs(32, seed=n + S)
+    ops = _abi.Ops(codes, coeffs)
+    psi = syn.random_states(1 << n, S, seed=n * S).astype(cdt)
+    assert _abi.op_expectation_plan(ops, dtype, 1 << n, S)["shared_groups"] > 0
+    monkeypatch.setenv("FAST_PAULI_EV_PACK", "1")
+    packed = H.op_expectation(ops, psi, dtype)
+    monkeypatch.setenv("FAST_PAULI_EV_PACK", "2")
+    unpacked = H.op_expectation(ops, psi, dtype)
+    assert np.array_equal(packed, unpacked)
+    exp = O.op_expectation(syn.codes_to_strings(codes), coeffs.astype(cdt).astype(np.complex128),
+                           psi.astype(np.complex128), np.zeros(S, np.complex128))
+    tol = 1e-12 if dtype == C128 else 1e-5
+    assert H.ev_scaled_err(packed, exp, coeffs, psi) <= tol
 
 
 @pytest.mark.parametrize("dtype", [C128, C64])
