@@ -802,14 +802,18 @@ __global__ __launch_bounds__(TKB * 8, WIDE ? 2 : 4) void k_op_expect_tiled(typen
         cases = __builtin_amdgcn_readfirstlane(cases);
         zis = __builtin_amdgcn_readfirstlane(zis);
         auto compute = [&](uint64_t q) {
-            // complex64: per-tile copies the compiler must re-read, so that the per-string
-            // sign factors and string-count tests are derived each tile in scalar registers
-            // instead of being held across the loop, where they spilled and pushed vector
-            // registers into scratch (n = 24, 16 states: 9.21 -> 8.87 ms).  Complex128 keeps
-            // them hoisted (18.50 against 18.58 ms re-read).
+            // Per-tile copies the compiler must re-read, so that what is derived from them
+            // is recomputed each tile in scalar registers instead of being held across the
+            // loop, where it spilled to vector lanes.  Complex64: the per-string sign factors
+            // and string-count tests (its string loop had pushed vector registers into
+            // scratch; n = 24, 16 states: 9.21 -> 8.87 ms).  Complex128: the string-count
+            // tests only (18.67 -> 18.60 ms; re-reading the sign factors too: 18.58 against
+            // 18.50 ms without either).
             uint32_t zis_t = zis, nsh_t = n_sh;
             if constexpr (sizeof(V) == 8)
                 asm volatile("" : "+s"(zis_t), "+s"(nsh_t));
+            else
+                asm volatile("" : "+s"(nsh_t));
             // coset signs of every string
             uint32_t cneg = 0;
             if constexpr (PK)
